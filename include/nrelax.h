@@ -1,0 +1,172 @@
+/*
+ * nrelax.h -- C ABI of the MI355X network relaxation library (libnrelax.so).
+ *
+ * The library relaxes 2-D bond-stretch + fiber-bend networks under imposed
+ * shear/compression by FIRE (and nonlinear CG) on one gfx950 GPU.  It is the
+ * drop-in for the relax step of the reference program `lsrelax`
+ * (src/large_strain_relax.cu).  Plain pointers and sizes only; every function
+ * returns an int status (NR_OK == 0) unless stated otherwise, and
+ * nr_last_error() gives a message for the calling thread's last failure.
+ *
+ * Handles are thread-compatible: use one handle from one thread at a time.
+ */
+#ifndef NRELAX_H
+#define NRELAX_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------ */
+#define NR_OK 0
+#define NR_E_OPEN 1      /* file cannot be opened */
+#define NR_E_HEADER 2    /* .dat header invalid (reference :60-65) */
+#define NR_E_POINTS 3    /* too few points (reference :81-85) */
+#define NR_E_INDEX 4     /* bond index out of range (reference :100-104) */
+#define NR_E_EDGES 5     /* too few edges (reference :111) */
+#define NR_E_ALLOC 6     /* host or device allocation failed */
+#define NR_E_ARG 7       /* invalid argument */
+#define NR_E_HIP 8       /* HIP runtime error (no GPU, launch failure, ...) */
+#define NR_E_TOPO 9      /* topology the device layout cannot hold */
+#define NR_E_BATCH 10    /* batch file invalid (reference :297-383) */
+#define NR_E_TIMEOUT 11  /* a device wait exceeded its bound */
+
+/* ---- per-relax outcome (nr_report.status) ----------------------------- */
+#define NR_CONVERGED 0
+#define NR_NOT_CONVERGED 1   /* step limit reached (logged, not an error) */
+#define NR_NONFINITE 2       /* force norm became NaN/Inf */
+#define NR_ABORTED 3         /* device wait bound exceeded */
+
+#define NR_CRIT_RMS 0        /* |F|_2 / sqrt(2N) < fcut (reference :716) */
+#define NR_CRIT_MAX 1        /* max over nodes of |F_node| < fcut */
+
+#define NR_METHOD_FIRE 0
+#define NR_METHOD_CG 1
+
+#define NR_PREC_FP64 0       /* FP64 throughout */
+#define NR_PREC_MIXED 1      /* FP32 forces on FP64 anchor + FP32 displacement,
+                                FP64 reductions, FP64 polish */
+
+typedef struct nr_network nr_network;  /* host-side network + topology */
+typedef struct nr_batch nr_batch;      /* device-resident set of networks */
+
+typedef struct {
+    int n_nodes, n_bonds, n_triples;
+    double width;      /* x period (reference .dat header "offset") */
+    double height;     /* y period; 0 = reference mode (top/bottom clamped) */
+    double miny, maxy; /* extreme y of the input geometry */
+} nr_network_info;
+
+/* FIRE / CG parameters: the reference's batch lines 3 and 4
+ * (large_strain_relax.cu:334-354), plus criterion and method. */
+typedef struct {
+    double alpha0, f_alpha, dt_max, f_inc, f_dec; /* fire_params[5] */
+    int nmin;
+    double mass;
+    double fcut;
+    int max_steps;
+    int criterion;     /* NR_CRIT_RMS or NR_CRIT_MAX */
+    int method;        /* NR_METHOD_FIRE or NR_METHOD_CG */
+} nr_relax_params;
+
+/* Result of one relax: report_data[3] and step_count of run_fire_md
+ * (large_strain_relax.cu:596, :721, :743, :751) plus max force and status. */
+typedef struct {
+    double rms_force;  /* |F|_2 / sqrt(2N) */
+    double max_force;  /* max node |F| */
+    double e_stretch;  /* sum 1/2 ks (d - L)^2 */
+    double e_bend;     /* sum 1/2 kb (theta - pi)^2 */
+    int steps;         /* steps taken, the final one included */
+    int status;        /* NR_CONVERGED, NR_NOT_CONVERGED, ... */
+    int fp64_steps;    /* mixed precision: steps of the FP64 polish */
+    int pad;
+} nr_report;
+
+typedef struct {
+    int precision;     /* NR_PREC_FP64 / NR_PREC_MIXED */
+    int block;         /* threads per workgroup, 0 = tuned */
+    int tile_nodes;    /* nodes per tile, 0 = tuned */
+    int team;          /* workgroups per network, 0 = tuned */
+    int device;        /* HIP device ordinal */
+    int pad;
+} nr_batch_opts;
+
+/* ---- errors ------------------------------------------------------------ */
+const char *nr_last_error(void);
+const char *nr_version(void);
+
+/* ---- host-side networks (no GPU needed) -------------------------------- */
+
+/* read_network (large_strain_relax.cu:42-114): header "N E width [height]",
+ * N point lines "x y", E edge lines "i j wx [wy]"; rest lengths from the
+ * input geometry.  A 4th header number selects the Lees-Edwards box. */
+int nr_network_read(const char *path, nr_network **out);
+int nr_network_from_arrays(int n_nodes, int n_bonds, double width,
+                           double height, const double *pos, const int *bonds,
+                           const signed char *wraps, nr_network **out);
+/* Seeded triangular lattice L x L (spacing 1), bond kept with probability p. */
+int nr_network_lattice(int L, double p, uint64_t seed, int lees_edwards,
+                       nr_network **out);
+/* get_triples (large_strain_relax.cu:116-184). */
+int nr_network_build_triples(nr_network *net);
+/* Per-fiber bending stiffness, log-uniform in [lo, hi] from seed. */
+int nr_network_set_fiber_kappa(nr_network *net, double lo, double hi,
+                               uint64_t seed);
+int nr_network_get_info(const nr_network *net, nr_network_info *info);
+/* Copies out any non-NULL array: pos[2N], bonds[2E], bond_wraps[2E],
+ * rest[E], triples[3T], triple_wraps[4T], triple_kappa[T] (zeros when
+ * uniform), fcodes[N] (clamp codes, reference :896-902). */
+int nr_network_get_arrays(const nr_network *net, double *pos, int *bonds,
+                          signed char *bond_wraps, double *rest, int *triples,
+                          signed char *triple_wraps, double *triple_kappa,
+                          unsigned char *fcodes);
+int nr_network_write_dat(const nr_network *net, const char *path);
+void nr_network_free(nr_network *net);
+
+/* ---- device batches ----------------------------------------------------- */
+
+/* Uploads `count` networks (triples must already be built when kb > 0) with
+ * stiffnesses ks[i], kb[i].  Positions start at each network's input
+ * geometry.  opts may be NULL (FP64, tuned shapes, device 0). */
+int nr_batch_create(int count, nr_network *const *nets, const double *ks,
+                    const double *kb, const nr_batch_opts *opts,
+                    nr_batch **out);
+/* Affine step of affine_displace (large_strain_relax.cu:527-537) for one
+ * network (index) or all (index < 0): x += xdisp (y - miny)/yrange,
+ * y += ydisp (y - miny)/yrange, applied at the start of the next relax, fused
+ * into its first force pass.  yrange = maxy - miny (reference mode) or the box
+ * height (Lees-Edwards, where xdisp also advances the image shift). */
+int nr_batch_strain(nr_batch *b, int index, double xdisp, double ydisp);
+/* Relaxes every network of the batch (run_fire_md, :596-762) in one
+ * persistent launch; params is one struct per network, or one shared struct
+ * when shared != 0.  reports[count] receives the outcomes. */
+int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared,
+                   nr_report *reports);
+int nr_batch_get_positions(nr_batch *b, int index, double *pos);
+int nr_batch_set_positions(nr_batch *b, int index, const double *pos);
+/* One force / energy evaluation at the current positions (pending strain
+ * applied first), for checking: force[2N], energies[2] = {stretch, bend}. */
+int nr_batch_eval(nr_batch *b, int index, double *force, double *energies);
+/* Device time of the last relax launch in milliseconds. */
+double nr_batch_last_ms(const nr_batch *b);
+/* Bytes one FIRE step moves per the algorithmic count for this batch. */
+int nr_batch_counts(const nr_batch *b, int index, int64_t *nodes,
+                    int64_t *bonds, int64_t *triples);
+void nr_batch_destroy(nr_batch *b);
+
+/* ---- the reference program's relax protocol ---------------------------- */
+
+/* relaxation_batch + do_relaxation_run (large_strain_relax.cu:792-1044):
+ * parses the 6-line batch file, finds <base>*.dat, and for each network runs
+ * the compression then shear ramps, writing <m>_<tag>_log.txt,
+ * <m>_<tag>_edata.txt and <m>_<tag>_<NNN>.pos as the reference does.
+ * strain_mode != 0 scales displacements by the y range (the default). */
+int nr_relaxation_batch(const char *batch_file, int strain_mode,
+                        const nr_batch_opts *opts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

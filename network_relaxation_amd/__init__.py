@@ -1,0 +1,259 @@
+"""MI355X network relaxation: Python view of libnrelax.so's C ABI.
+
+The product is the C ABI in include/nrelax.h (libnrelax.so, HIP kernels for
+gfx950) and the `nrelax` CLI, a drop-in for the reference's `lsrelax -b`.
+This module binds the C ABI with ctypes for tests and benchmarks.  Nothing
+here computes on the CPU: every force, energy and relax runs in the library,
+and importing raises if the library has not been built.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnrelax.so")
+CLI_PATH = os.path.join(_HERE, "nrelax")
+
+OK = 0
+E_OPEN, E_HEADER, E_POINTS, E_INDEX, E_EDGES, E_ALLOC, E_ARG, E_HIP, E_TOPO, E_BATCH, E_TIMEOUT = range(1, 12)
+CONVERGED, NOT_CONVERGED, NONFINITE, ABORTED = range(4)
+CRIT_RMS, CRIT_MAX = 0, 1
+METHOD_FIRE, METHOD_CG = 0, 1
+PREC_FP64, PREC_MIXED = 0, 1
+
+
+class NetworkInfo(C.Structure):
+    _fields_ = [("n_nodes", C.c_int), ("n_bonds", C.c_int), ("n_triples", C.c_int),
+                ("width", C.c_double), ("height", C.c_double),
+                ("miny", C.c_double), ("maxy", C.c_double)]
+
+
+class RelaxParams(C.Structure):
+    _fields_ = [("alpha0", C.c_double), ("f_alpha", C.c_double), ("dt_max", C.c_double),
+                ("f_inc", C.c_double), ("f_dec", C.c_double), ("nmin", C.c_int),
+                ("mass", C.c_double), ("fcut", C.c_double), ("max_steps", C.c_int),
+                ("criterion", C.c_int), ("method", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("rms_force", C.c_double), ("max_force", C.c_double),
+                ("e_stretch", C.c_double), ("e_bend", C.c_double),
+                ("steps", C.c_int), ("status", C.c_int), ("fp64_steps", C.c_int),
+                ("pad", C.c_int)]
+
+    def as_dict(self):
+        return dict(rms=self.rms_force, fmax=self.max_force, es=self.e_stretch,
+                    eb=self.e_bend, steps=self.steps, status=self.status,
+                    fp64_steps=self.fp64_steps)
+
+
+class BatchOpts(C.Structure):
+    _fields_ = [("precision", C.c_int), ("block", C.c_int), ("tile_nodes", C.c_int),
+                ("team", C.c_int), ("device", C.c_int), ("pad", C.c_int)]
+
+
+class NRError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"nrelax error {code}: {msg}")
+        self.code = code
+
+
+_lib = None
+
+
+def lib():
+    """Loads libnrelax.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build with `make -C network_relaxation_amd/csrc`")
+        L = C.CDLL(LIB_PATH)
+        P, vp, dp, ip = C.POINTER, C.c_void_p, C.POINTER(C.c_double), C.POINTER(C.c_int)
+        L.nr_last_error.restype = C.c_char_p
+        L.nr_version.restype = C.c_char_p
+        L.nr_network_read.argtypes = [C.c_char_p, P(vp)]
+        L.nr_network_from_arrays.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, dp, ip,
+                                             P(C.c_byte), P(vp)]
+        L.nr_network_lattice.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_int, P(vp)]
+        L.nr_network_build_triples.argtypes = [vp]
+        L.nr_network_set_fiber_kappa.argtypes = [vp, C.c_double, C.c_double, C.c_uint64]
+        L.nr_network_get_info.argtypes = [vp, P(NetworkInfo)]
+        L.nr_network_get_arrays.argtypes = [vp, dp, ip, P(C.c_byte), dp, ip, P(C.c_byte), dp,
+                                            P(C.c_ubyte)]
+        L.nr_network_write_dat.argtypes = [vp, C.c_char_p]
+        L.nr_network_free.argtypes = [vp]
+        L.nr_network_free.restype = None
+        L.nr_batch_create.argtypes = [C.c_int, P(vp), dp, dp, P(BatchOpts), P(vp)]
+        L.nr_batch_strain.argtypes = [vp, C.c_int, C.c_double, C.c_double]
+        L.nr_batch_relax.argtypes = [vp, P(RelaxParams), C.c_int, P(Report)]
+        L.nr_batch_get_positions.argtypes = [vp, C.c_int, dp]
+        L.nr_batch_set_positions.argtypes = [vp, C.c_int, dp]
+        L.nr_batch_eval.argtypes = [vp, C.c_int, dp, dp]
+        L.nr_batch_last_ms.argtypes = [vp]
+        L.nr_batch_last_ms.restype = C.c_double
+        L.nr_batch_counts.argtypes = [vp, C.c_int, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
+        L.nr_batch_destroy.argtypes = [vp]
+        L.nr_batch_destroy.restype = None
+        L.nr_relaxation_batch.argtypes = [C.c_char_p, C.c_int, P(BatchOpts)]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise NRError(rc, lib().nr_last_error().decode())
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Network:
+    """Host-side network (nr_network): .dat reader, lattice, triples."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def read(cls, path):
+        h = C.c_void_p()
+        _check(lib().nr_network_read(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def lattice(cls, L, p, seed, lees_edwards=True):
+        h = C.c_void_p()
+        _check(lib().nr_network_lattice(L, p, seed, 1 if lees_edwards else 0, C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def from_arrays(cls, pos, bonds, wraps, width, height=0.0):
+        pos = np.ascontiguousarray(pos, np.float64).reshape(-1)
+        bonds = np.ascontiguousarray(bonds, np.int32).reshape(-1)
+        wraps = np.ascontiguousarray(wraps, np.int8).reshape(-1)
+        h = C.c_void_p()
+        _check(lib().nr_network_from_arrays(len(pos) // 2, len(bonds) // 2, width, height,
+                                            _dp(pos), bonds.ctypes.data_as(C.POINTER(C.c_int)),
+                                            wraps.ctypes.data_as(C.POINTER(C.c_byte)),
+                                            C.byref(h)))
+        return cls(h.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.nr_network_free(self.h)
+            self.h = None
+
+    def triples(self):
+        _check(lib().nr_network_build_triples(self.h))
+        return self
+
+    def fiber_kappa(self, lo, hi, seed):
+        _check(lib().nr_network_set_fiber_kappa(self.h, lo, hi, seed))
+        return self
+
+    def info(self):
+        i = NetworkInfo()
+        _check(lib().nr_network_get_info(self.h, C.byref(i)))
+        return i
+
+    def arrays(self):
+        i = self.info()
+        n, e, t = i.n_nodes, i.n_bonds, i.n_triples
+        pos = np.zeros(2 * n)
+        bonds = np.zeros(2 * max(e, 1), np.int32)
+        bw = np.zeros(2 * max(e, 1), np.int8)
+        rest = np.zeros(max(e, 1))
+        tri = np.zeros(3 * max(t, 1), np.int32)
+        tw = np.zeros(4 * max(t, 1), np.int8)
+        tk = np.zeros(max(t, 1))
+        fc = np.zeros(n, np.uint8)
+        _check(lib().nr_network_get_arrays(
+            self.h, _dp(pos), bonds.ctypes.data_as(C.POINTER(C.c_int)),
+            bw.ctypes.data_as(C.POINTER(C.c_byte)), _dp(rest),
+            tri.ctypes.data_as(C.POINTER(C.c_int)), tw.ctypes.data_as(C.POINTER(C.c_byte)),
+            _dp(tk), fc.ctypes.data_as(C.POINTER(C.c_ubyte))))
+        return dict(pos=pos, bonds=bonds[:2 * e].reshape(-1, 2), bond_wraps=bw[:2 * e].reshape(-1, 2),
+                    rest=rest[:e], triples=tri[:3 * t].reshape(-1, 3),
+                    triple_wraps=tw[:4 * t].reshape(-1, 4), triple_kappa=tk[:t], fcodes=fc)
+
+    def write_dat(self, path):
+        _check(lib().nr_network_write_dat(self.h, os.fsencode(path)))
+
+
+def relax_params(alpha0=0.1, f_alpha=0.99, dt_max=0.1, f_inc=1.1, f_dec=0.5, nmin=5, mass=1.0,
+                 fcut=1e-10, max_steps=1000000, criterion=CRIT_MAX, method=METHOD_FIRE):
+    return RelaxParams(alpha0, f_alpha, dt_max, f_inc, f_dec, nmin, mass, fcut, max_steps,
+                       criterion, method)
+
+
+class Batch:
+    """Device-resident networks (nr_batch) relaxed by one persistent launch."""
+
+    def __init__(self, networks, ks, kb, precision=PREC_FP64, tile_nodes=0, team=0, device=0):
+        self.networks = list(networks)  # keep the host networks alive
+        n = len(self.networks)
+        ks = np.broadcast_to(np.asarray(ks, np.float64), (n,)).copy()
+        kb = np.broadcast_to(np.asarray(kb, np.float64), (n,)).copy()
+        handles = (C.c_void_p * n)(*[x.h.value for x in self.networks])
+        opts = BatchOpts(precision, 0, tile_nodes, team, device, 0)
+        h = C.c_void_p()
+        _check(lib().nr_batch_create(n, handles, _dp(ks), _dp(kb), C.byref(opts), C.byref(h)))
+        self.h = h
+        self.count = n
+        self.n_nodes = [x.info().n_nodes for x in self.networks]
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.nr_batch_destroy(self.h)
+            self.h = None
+
+    def strain(self, index, xdisp, ydisp=0.0):
+        _check(lib().nr_batch_strain(self.h, index, xdisp, ydisp))
+
+    def shear(self, index, gamma):
+        """Shear strain increment gamma (strain mode: displacement gamma*yrange)."""
+        idx = range(self.count) if index < 0 else [index]
+        for k in idx:
+            i = self.networks[k].info()
+            yr = i.height if i.height > 0 else i.maxy - i.miny
+            _check(lib().nr_batch_strain(self.h, k, gamma * yr, 0.0))
+
+    def relax(self, params):
+        reps = (Report * self.count)()
+        if isinstance(params, RelaxParams):
+            _check(lib().nr_batch_relax(self.h, C.byref(params), 1, reps))
+        else:
+            arr = (RelaxParams * self.count)(*params)
+            _check(lib().nr_batch_relax(self.h, arr, 0, reps))
+        return [r.as_dict() for r in reps]
+
+    def positions(self, index):
+        out = np.zeros(2 * self.n_nodes[index])
+        _check(lib().nr_batch_get_positions(self.h, index, _dp(out)))
+        return out
+
+    def set_positions(self, index, pos):
+        pos = np.ascontiguousarray(pos, np.float64)
+        _check(lib().nr_batch_set_positions(self.h, index, _dp(pos)))
+
+    def eval(self, index):
+        f = np.zeros(2 * self.n_nodes[index])
+        e = np.zeros(2)
+        _check(lib().nr_batch_eval(self.h, index, _dp(f), _dp(e)))
+        return f, e
+
+    def last_ms(self):
+        return lib().nr_batch_last_ms(self.h)
+
+    def counts(self, index=-1):
+        n, e, t = C.c_int64(), C.c_int64(), C.c_int64()
+        _check(lib().nr_batch_counts(self.h, index, C.byref(n), C.byref(e), C.byref(t)))
+        return n.value, e.value, t.value
+
+
+def relaxation_batch(batch_file, strain_mode=True, precision=PREC_FP64, tile_nodes=0):
+    """nr_relaxation_batch: the reference program's whole protocol."""
+    opts = BatchOpts(precision, 0, tile_nodes, 0, 0, 0)
+    return lib().nr_relaxation_batch(os.fsencode(batch_file), 1 if strain_mode else 0,
+                                     C.byref(opts))

@@ -1,0 +1,526 @@
+// C ABI of libnrelax.so: host networks, device batches and the launch of the
+// persistent relax kernel.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/nrelax.h"
+#include "device.hpp"
+#include "network.hpp"
+#include "plan.hpp"
+
+struct nr_network {
+    nr::Network net;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+int hip_fail(hipError_t e, const char *what)
+{
+    g_err = std::string(what) + ": " + hipGetErrorString(e);
+    return NR_E_HIP;
+}
+
+#define HIP_TRY(expr)                                  \
+    do {                                               \
+        hipError_t e_ = (expr);                        \
+        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
+    } while (0)
+
+template <class T>
+struct DevBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    ~DevBuf()
+    {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t count)
+    {
+        n = count;
+        return hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(count, 1));
+    }
+    hipError_t upload(const std::vector<T> &h)
+    {
+        hipError_t e = alloc(h.size());
+        if (e != hipSuccess || h.empty()) return e;
+        return hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
+    }
+};
+
+}  // namespace
+
+struct nr_batch {
+    int device = 0;
+    int precision = NR_PREC_FP64;
+    int team = 1, grid = 0, npt = 1, block = 256;
+    bool xcd_local = true, cooperative = false;
+    size_t lds = 0;
+    nr::Plan plan;
+    std::vector<const nr::Network *> nets;  // borrowed for heights / miny
+    std::vector<nr::NetDev> hnet;
+    std::vector<double> ks, kb;
+    std::vector<int> jobs;
+    DevBuf<double2> x[2], v[2], f[2];
+    DevBuf<unsigned char> mask;
+    DevBuf<int> brow, trow, halo, djobs, queue, mailbox;
+    DevBuf<uint32_t> bcode;
+    DevBuf<double> blen, tkb, partials;
+    DevBuf<uint64_t> tcode;
+    DevBuf<nr::TileDev> tiles;
+    DevBuf<nr::NetDev> dnet;
+    DevBuf<nr::NetOut> dout;
+    DevBuf<unsigned> bar, err;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_ms = 0.0;
+    int n_teams = 0;
+
+    ~nr_batch()
+    {
+        if (ev0) (void)hipEventDestroy(ev0);
+        if (ev1) (void)hipEventDestroy(ev1);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+extern "C" {
+
+const char *nr_last_error(void) { return g_err.c_str(); }
+
+const char *nr_version(void) { return "nrelax 0.1 (gfx950)"; }
+
+int nr_network_read(const char *path, nr_network **out)
+{
+    if (!path || !out) return fail(NR_E_ARG, "null argument");
+    auto h = std::make_unique<nr_network>();
+    std::string err;
+    int rc = nr::read_network(path, h->net, err);
+    if (rc) return fail(rc, err);
+    *out = h.release();
+    return NR_OK;
+}
+
+int nr_network_from_arrays(int n_nodes, int n_bonds, double width, double height,
+                           const double *pos, const int *bonds, const signed char *wraps,
+                           nr_network **out)
+{
+    if (!out) return fail(NR_E_ARG, "null argument");
+    auto h = std::make_unique<nr_network>();
+    std::string err;
+    int rc = nr::network_from_arrays(n_nodes, n_bonds, width, height, pos, bonds, wraps,
+                                     h->net, err);
+    if (rc) return fail(rc, err);
+    *out = h.release();
+    return NR_OK;
+}
+
+int nr_network_lattice(int L, double p, uint64_t seed, int lees_edwards, nr_network **out)
+{
+    if (!out) return fail(NR_E_ARG, "null argument");
+    auto h = std::make_unique<nr_network>();
+    std::string err;
+    int rc = nr::make_lattice(L, p, seed, lees_edwards != 0, h->net, err);
+    if (rc) return fail(rc, err);
+    *out = h.release();
+    return NR_OK;
+}
+
+int nr_network_build_triples(nr_network *net)
+{
+    if (!net) return fail(NR_E_ARG, "null network");
+    return nr::build_triples(net->net);
+}
+
+int nr_network_set_fiber_kappa(nr_network *net, double lo, double hi, uint64_t seed)
+{
+    if (!net) return fail(NR_E_ARG, "null network");
+    int rc = nr::set_fiber_kappa(net->net, lo, hi, seed);
+    if (rc) return fail(rc, "fiber kappa needs built triples and 0 < lo <= hi");
+    return NR_OK;
+}
+
+int nr_network_get_info(const nr_network *net, nr_network_info *info)
+{
+    if (!net || !info) return fail(NR_E_ARG, "null argument");
+    info->n_nodes = net->net.n;
+    info->n_bonds = net->net.e();
+    info->n_triples = net->net.t();
+    info->width = net->net.width;
+    info->height = net->net.height;
+    info->miny = net->net.miny;
+    info->maxy = net->net.maxy;
+    return NR_OK;
+}
+
+int nr_network_get_arrays(const nr_network *net, double *pos, int *bonds,
+                          signed char *bond_wraps, double *rest, int *triples,
+                          signed char *triple_wraps, double *triple_kappa,
+                          unsigned char *fcodes)
+{
+    if (!net) return fail(NR_E_ARG, "null network");
+    const nr::Network &n = net->net;
+    if (pos) std::copy(n.pos.begin(), n.pos.end(), pos);
+    if (bonds) std::copy(n.bonds.begin(), n.bonds.end(), bonds);
+    if (bond_wraps) std::copy(n.bwrap.begin(), n.bwrap.end(), bond_wraps);
+    if (rest) std::copy(n.rest.begin(), n.rest.end(), rest);
+    if (triples) std::copy(n.tri.begin(), n.tri.end(), triples);
+    if (triple_wraps) std::copy(n.twrap.begin(), n.twrap.end(), triple_wraps);
+    if (triple_kappa) {
+        if (n.tkb.empty()) std::fill(triple_kappa, triple_kappa + n.t(), 0.0);
+        else std::copy(n.tkb.begin(), n.tkb.end(), triple_kappa);
+    }
+    if (fcodes) std::copy(n.fcode.begin(), n.fcode.end(), fcodes);
+    return NR_OK;
+}
+
+int nr_network_write_dat(const nr_network *net, const char *path)
+{
+    if (!net || !path) return fail(NR_E_ARG, "null argument");
+    int rc = nr::write_dat(net->net, path);
+    if (rc) return fail(rc, std::string("cannot write ") + path);
+    return NR_OK;
+}
+
+void nr_network_free(nr_network *net) { delete net; }
+
+// ---- batches --------------------------------------------------------------
+
+static int choose_shapes(nr_batch *b, const nr_batch_opts *opts)
+{
+    // nodes per tile: 512 by default (2 per thread of a 256-thread workgroup)
+    int tile = (opts && opts->tile_nodes > 0) ? opts->tile_nodes : 512;
+    tile = std::min(tile, 2048);
+    b->npt = 1;
+    while (b->npt * 256 < tile) b->npt *= 2;
+    if (b->npt > 8) return fail(NR_E_ARG, "tile_nodes above 2048");
+    b->plan.tile_nodes = tile;
+    return NR_OK;
+}
+
+int nr_batch_create(int count, nr_network *const *nets, const double *ks, const double *kb,
+                    const nr_batch_opts *opts, nr_batch **out)
+{
+    if (count <= 0 || !nets || !ks || !kb || !out) return fail(NR_E_ARG, "invalid arguments");
+    auto b = std::make_unique<nr_batch>();
+    b->device = opts ? opts->device : 0;
+    b->precision = opts ? opts->precision : NR_PREC_FP64;
+    if (b->precision != NR_PREC_FP64) return fail(NR_E_ARG, "precision not available");
+    HIP_TRY(hipSetDevice(b->device));
+    int rc = choose_shapes(b.get(), opts);
+    if (rc) return rc;
+    for (int i = 0; i < count; i++) {
+        if (!nets[i]) return fail(NR_E_ARG, "null network in batch");
+        if (!(ks[i] >= 0) || !(kb[i] >= 0)) return fail(NR_E_ARG, "negative stiffness");
+        b->nets.push_back(&nets[i]->net);
+        b->ks.push_back(ks[i]);
+        b->kb.push_back(kb[i]);
+    }
+    std::string err;
+    rc = nr::build_plan(b->nets, b->ks, b->kb, b->plan.tile_nodes, 64 * 1024, b->plan, err);
+    if (rc) return fail(rc, err);
+    const nr::Plan &P = b->plan;
+    b->lds = (size_t)P.lds_bytes();
+
+    // team shape: one network -> the whole GPU; many -> one workgroup each
+    int per_cu = 0, n_cu = 0;
+    HIP_TRY((hipError_t)nr::max_resident_blocks(b->npt, 256, b->lds, &per_cu, &n_cu));
+    if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
+    const int resident = per_cu * n_cu;
+    int team = opts && opts->team > 0 ? opts->team : 0;
+    if (team == 0) {
+        if (count == 1) {
+            int tiles = P.nets[0].n_tiles;
+            team = std::max(1, std::min(tiles, resident));
+        } else {
+            team = 1;
+        }
+    }
+    b->team = team;
+    if (team > 1 && count == 1) {
+        b->xcd_local = false;
+        b->grid = team;
+        b->n_teams = 1;
+    } else if (team > 1) {
+        b->xcd_local = true;
+        int per = std::max(1, resident / (8 * team));
+        b->grid = 8 * team * per;
+        b->n_teams = b->grid / team;
+    } else {
+        b->xcd_local = false;
+        b->grid = std::max(1, std::min(resident, count));
+        b->n_teams = b->grid;
+    }
+    if (b->grid > resident && team > 1) return fail(NR_E_ARG, "team exceeds resident workgroups");
+    b->cooperative = team > 1;
+
+    // uploads
+    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    HIP_TRY(hipEventCreate(&b->ev0));
+    HIP_TRY(hipEventCreate(&b->ev1));
+    const size_t N = (size_t)P.n_total;
+    std::vector<double2> x0(N);
+    for (int k = 0; k < count; k++) {
+        const nr::PlanNet &pn = P.nets[k];
+        const nr::Network &net = *b->nets[k];
+        for (int d = 0; d < pn.n_nodes; d++) {
+            int o = pn.perm[d];
+            x0[pn.node_base + d] = make_double2(net.pos[2 * o], net.pos[2 * o + 1]);
+        }
+    }
+    std::vector<double2> zeros(N, make_double2(0.0, 0.0));
+    for (int s = 0; s < 2; s++) {
+        HIP_TRY(b->x[s].upload(x0));
+        HIP_TRY(b->v[s].upload(zeros));
+        HIP_TRY(b->f[s].upload(zeros));
+    }
+    HIP_TRY(b->mask.upload(P.mask));
+    HIP_TRY(b->brow.upload(P.brow));
+    HIP_TRY(b->trow.upload(P.trow));
+    HIP_TRY(b->halo.upload(P.halo));
+    HIP_TRY(b->bcode.upload(P.bcode));
+    HIP_TRY(b->blen.upload(P.blen));
+    HIP_TRY(b->tcode.upload(P.tcode));
+    HIP_TRY(b->tkb.upload(P.tkb));
+    HIP_TRY(b->tiles.upload(P.tiles));
+    b->hnet.resize(count);
+    for (int k = 0; k < count; k++) {
+        nr::NetDev &d = b->hnet[k];
+        std::memset(&d, 0, sizeof(d));
+        const nr::PlanNet &pn = P.nets[k];
+        const nr::Network &net = *b->nets[k];
+        d.node_base = pn.node_base;
+        d.n_nodes = pn.n_nodes;
+        d.tile_base = pn.tile_base;
+        d.n_tiles = pn.n_tiles;
+        d.width = net.width;
+        d.height = net.height;
+        d.miny = net.miny;
+        d.yrange = net.height > 0 ? net.height : net.maxy - net.miny;
+        d.ks = b->ks[k];
+        d.kb = b->kb[k];
+    }
+    HIP_TRY(b->dnet.upload(b->hnet));
+    HIP_TRY(b->dout.alloc(count));
+    // jobs: largest first (cost ~ N^1.5)
+    b->jobs.resize(count);
+    for (int k = 0; k < count; k++) b->jobs[k] = k;
+    std::stable_sort(b->jobs.begin(), b->jobs.end(), [&](int a, int c) {
+        return P.nets[a].n_nodes > P.nets[c].n_nodes;
+    });
+    HIP_TRY(b->djobs.upload(b->jobs));
+    HIP_TRY(b->queue.alloc(1));
+    HIP_TRY(b->partials.alloc((size_t)b->n_teams * b->team * 4));
+    HIP_TRY(b->bar.alloc((size_t)b->n_teams * 64));
+    HIP_TRY(hipMemset(b->bar.p, 0, sizeof(unsigned) * b->n_teams * 64));
+    HIP_TRY(b->mailbox.alloc((size_t)b->n_teams * 32));
+    HIP_TRY(b->err.alloc(1));
+    HIP_TRY(hipMemset(b->err.p, 0, sizeof(unsigned)));
+    *out = b.release();
+    return NR_OK;
+}
+
+int nr_batch_strain(nr_batch *b, int index, double xdisp, double ydisp)
+{
+    if (!b) return fail(NR_E_ARG, "null batch");
+    int lo = index < 0 ? 0 : index, hi = index < 0 ? (int)b->hnet.size() : index + 1;
+    if (index >= (int)b->hnet.size()) return fail(NR_E_ARG, "index out of range");
+    for (int k = lo; k < hi; k++) {
+        if (b->hnet[k].height > 0 && ydisp != 0.0)
+            return fail(NR_E_ARG, "compression is not defined in the Lees-Edwards box");
+        b->hnet[k].pend_x += xdisp;
+        b->hnet[k].pend_y += ydisp;
+    }
+    return NR_OK;
+}
+
+static int run_kernel(nr_batch *b, int mode)
+{
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipMemcpyAsync(b->dnet.p, b->hnet.data(), sizeof(nr::NetDev) * b->hnet.size(),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipMemsetAsync(b->queue.p, 0, sizeof(int), b->stream));
+    nr::KArgs a;
+    std::memset(&a, 0, sizeof(a));
+    for (int s = 0; s < 2; s++) {
+        a.x[s] = b->x[s].p;
+        a.v[s] = b->v[s].p;
+        a.f[s] = b->f[s].p;
+    }
+    a.mask = b->mask.p;
+    a.brow = b->brow.p;
+    a.trow = b->trow.p;
+    a.bcode = b->bcode.p;
+    a.blen = b->blen.p;
+    a.tcode = b->tcode.p;
+    a.tkb = b->tkb.p;
+    a.halo = b->halo.p;
+    a.tiles = b->tiles.p;
+    a.nets = b->dnet.p;
+    a.out = b->dout.p;
+    a.jobs = b->djobs.p;
+    a.n_jobs = (int)b->jobs.size();
+    a.team = b->team;
+    a.xcd_local = b->xcd_local ? 1 : 0;
+    a.has_tkb = b->plan.has_tkb ? 1 : 0;
+    a.max_local = b->plan.max_local;
+    a.max_nb = b->plan.max_nb;
+    a.max_nt = b->plan.max_nt;
+    a.mode = mode;
+    a.queue = b->queue.p;
+    a.partials = b->partials.p;
+    a.bar = b->bar.p;
+    a.mailbox = b->mailbox.p;
+    a.err = b->err.p;
+    a.timeout_ticks = 100000000ll * 60;  // 60 s at the 100 MHz constant clock
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    int e = nr::launch_relax(a, b->npt, b->grid, 256, b->lds, b->cooperative, b->stream);
+    if (e) return hip_fail((hipError_t)e, "relax kernel launch");
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    unsigned err = 0;
+    HIP_TRY(hipMemcpy(&err, b->err.p, sizeof(unsigned), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b->hnet.data(), b->dnet.p, sizeof(nr::NetDev) * b->hnet.size(),
+                      hipMemcpyDeviceToHost));
+    if (err) {
+        (void)hipMemset(b->err.p, 0, sizeof(unsigned));
+        return fail(NR_E_TIMEOUT, "a team barrier exceeded its wait bound");
+    }
+    return NR_OK;
+}
+
+int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared, nr_report *reports)
+{
+    if (!b || !params) return fail(NR_E_ARG, "null argument");
+    for (size_t k = 0; k < b->hnet.size(); k++) {
+        const nr_relax_params &p = shared ? params[0] : params[k];
+        if (p.method != NR_METHOD_FIRE) return fail(NR_E_ARG, "method not available");
+        if (!(p.mass > 0) || p.max_steps < 0 || p.fcut < 0)
+            return fail(NR_E_ARG, "invalid relax parameters");
+        nr::NetDev &d = b->hnet[k];
+        d.a0 = p.alpha0;
+        d.fa = p.f_alpha;
+        d.dtmax = p.dt_max;
+        d.finc = p.f_inc;
+        d.fdec = p.f_dec;
+        d.nmin = p.nmin;
+        d.mass = p.mass;
+        d.fcut = p.fcut;
+        d.max_steps = p.max_steps;
+        d.crit = p.criterion;
+        d.method = p.method;
+    }
+    int rc = run_kernel(b, nr::kModeRelax);
+    if (rc) return rc;
+    if (reports) {
+        std::vector<nr::NetOut> o(b->hnet.size());
+        HIP_TRY(hipMemcpy(o.data(), b->dout.p, sizeof(nr::NetOut) * o.size(),
+                          hipMemcpyDeviceToHost));
+        for (size_t k = 0; k < o.size(); k++) {
+            reports[k].rms_force = o[k].rms;
+            reports[k].max_force = o[k].fmax;
+            reports[k].e_stretch = o[k].es;
+            reports[k].e_bend = o[k].eb;
+            reports[k].steps = o[k].steps;
+            reports[k].status = o[k].status;
+            reports[k].fp64_steps = o[k].fp64_steps;
+            reports[k].pad = 0;
+        }
+    }
+    return NR_OK;
+}
+
+static int copy_state(nr_batch *b, int index, const DevBuf<double2> *bufs, double *out)
+{
+    if (!b || !out || index < 0 || index >= (int)b->hnet.size())
+        return fail(NR_E_ARG, "invalid arguments");
+    const nr::PlanNet &pn = b->plan.nets[index];
+    std::vector<double2> tmp(pn.n_nodes);
+    const int par = b->hnet[index].parity;
+    HIP_TRY(hipMemcpy(tmp.data(), bufs[par].p + pn.node_base, sizeof(double2) * pn.n_nodes,
+                      hipMemcpyDeviceToHost));
+    for (int d = 0; d < pn.n_nodes; d++) {
+        int o = pn.perm[d];
+        out[2 * o] = tmp[d].x;
+        out[2 * o + 1] = tmp[d].y;
+    }
+    return NR_OK;
+}
+
+int nr_batch_get_positions(nr_batch *b, int index, double *pos)
+{
+    return copy_state(b, index, b ? b->x : nullptr, pos);
+}
+
+int nr_batch_set_positions(nr_batch *b, int index, const double *pos)
+{
+    if (!b || !pos || index < 0 || index >= (int)b->hnet.size())
+        return fail(NR_E_ARG, "invalid arguments");
+    const nr::PlanNet &pn = b->plan.nets[index];
+    std::vector<double2> tmp(pn.n_nodes);
+    for (int d = 0; d < pn.n_nodes; d++) {
+        int o = pn.perm[d];
+        tmp[d] = make_double2(pos[2 * o], pos[2 * o + 1]);
+    }
+    const int par = b->hnet[index].parity;
+    HIP_TRY(hipMemcpy(b->x[par].p + pn.node_base, tmp.data(), sizeof(double2) * pn.n_nodes,
+                      hipMemcpyHostToDevice));
+    return NR_OK;
+}
+
+int nr_batch_eval(nr_batch *b, int index, double *force, double *energies)
+{
+    if (!b || index < 0 || index >= (int)b->hnet.size()) return fail(NR_E_ARG, "invalid arguments");
+    int rc = run_kernel(b, nr::kModeEval);
+    if (rc) return rc;
+    if (force) {
+        rc = copy_state(b, index, b->f, force);
+        if (rc) return rc;
+    }
+    if (energies) {
+        nr::NetOut o;
+        HIP_TRY(hipMemcpy(&o, b->dout.p + index, sizeof(o), hipMemcpyDeviceToHost));
+        energies[0] = o.es;
+        energies[1] = o.eb;
+    }
+    return NR_OK;
+}
+
+double nr_batch_last_ms(const nr_batch *b) { return b ? b->last_ms : 0.0; }
+
+int nr_batch_counts(const nr_batch *b, int index, int64_t *nodes, int64_t *bonds,
+                    int64_t *triples)
+{
+    if (!b) return fail(NR_E_ARG, "null batch");
+    int64_t n = 0, e = 0, t = 0;
+    for (size_t k = 0; k < b->plan.nets.size(); k++) {
+        if (index >= 0 && (int)k != index) continue;
+        n += b->plan.nets[k].n_nodes;
+        e += b->plan.nets[k].n_bonds;
+        t += b->plan.nets[k].n_triples;
+    }
+    if (nodes) *nodes = n;
+    if (bonds) *bonds = e;
+    if (triples) *triples = t;
+    return NR_OK;
+}
+
+void nr_batch_destroy(nr_batch *b) { delete b; }
+
+}  // extern "C"

@@ -1,0 +1,48 @@
+// nrelax: drop-in for the reference program `lsrelax`
+// (large_strain_relax.cu:1047-1126): nrelax -b <batch> [-a] [-i] [-x bx] [-y by]
+//   -b  batch file (6 lines, see parse_batch_file)
+//   -a  absolute displacements instead of strains (the reference's -a falls
+//       through into -b with no argument; here it does what it describes)
+//   -i  accepted and ignored, as in the reference
+//   -x/-y  the reference's block shape; here a hint for nodes per tile (x*y)
+#include <unistd.h>
+
+#include <cstdio>
+#include <cstdlib>
+
+#include "../../include/nrelax.h"
+
+int main(int argc, char **argv)
+{
+    const char *batch = nullptr;
+    int strain_mode = 1, bx = 0, by = 1, c;
+    while ((c = getopt(argc, argv, "ab:ix:y:")) != -1) {
+        switch (c) {
+        case 'a': strain_mode = 0; break;
+        case 'b': batch = optarg; break;
+        case 'i': break;
+        case 'x':
+            if (std::sscanf(optarg, "%d", &bx) < 1 || bx < 1) {
+                std::fprintf(stderr, "Block dimensions must be positive.\n");
+                bx = 0;
+            }
+            break;
+        case 'y':
+            if (std::sscanf(optarg, "%d", &by) < 1 || by < 1) {
+                std::fprintf(stderr, "Block dimensions must be positive.\n");
+                by = 1;
+            }
+            break;
+        default: break;
+        }
+    }
+    if (!batch) {
+        std::fprintf(stderr, "No batch file was specified.\n");
+        return 255;
+    }
+    nr_batch_opts opts = {};
+    opts.precision = NR_PREC_FP64;
+    opts.tile_nodes = bx > 0 ? bx * by : 0;
+    int rc = nr_relaxation_batch(batch, strain_mode, &opts);
+    return rc == NR_OK ? 0 : 1;
+}

@@ -1,0 +1,351 @@
+// The reference program's relax protocol on top of the C ABI:
+// parse_batch_file (large_strain_relax.cu:297-383), get_file_lists
+// (file_lists.cpp:80-137), do_relaxation_run (:792-1010) and
+// relaxation_batch (:1016-1044), with the same output files and formats.
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <ctime>
+#include <filesystem>
+#include <fstream>
+#include <regex>
+#include <sstream>
+#include <string>
+#include <vector>
+
+#include "../../include/nrelax.h"
+#include "protocol.hpp"
+
+namespace nr {
+
+namespace {
+
+constexpr double kFltTol = 1e-10;
+
+// read_line :263-283: skip blank and '#' lines.
+bool read_line(std::ifstream &in, std::string &line)
+{
+    if (in.eof()) return false;
+    while (!in.eof()) {
+        std::getline(in, line);
+        if (line.empty() || line[0] == '#') continue;
+        break;
+    }
+    return !(line.empty() || line[0] == '#');
+}
+
+std::vector<std::string> split_ws(const std::string &s)
+{
+    std::vector<std::string> out;
+    std::string cur;
+    for (char c : s) {
+        if (c == ' ' || c == '\t') {
+            if (!cur.empty()) out.push_back(cur);
+            cur.clear();
+        } else {
+            cur.push_back(c);
+        }
+    }
+    if (!cur.empty()) out.push_back(cur);
+    return out;
+}
+
+}  // namespace
+
+int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err)
+{
+    std::ifstream in(path);
+    std::string line;
+    b = BatchSpec();
+    if (!in.is_open()) {
+        err = "The file " + path + " could not be read.";
+        return NR_E_OPEN;
+    }
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    std::vector<std::string> tok = split_ws(line);
+    if (tok.empty()) { err = "missing base name"; return NR_E_BATCH; }
+    b.base = tok[0];
+    if (tok.size() > 1) b.tag = tok[1];
+
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    double klo = 0, khi = 0, seed = 0;
+    int nr = std::sscanf(line.c_str(), "%lf %lf %lf %lf %lf", &b.ks, &b.kb, &klo, &khi, &seed);
+    if (nr < 2 || b.ks < 0 || b.kb < 0) { err = "Error: invalid format for mechanical attributes."; return NR_E_BATCH; }
+    if (nr == 5 && klo > 0 && khi >= klo) {
+        b.kappa_lo = klo;
+        b.kappa_hi = khi;
+        b.kappa_seed = (uint64_t)seed;
+    }
+
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    int cnt = 0;
+    for (const std::string &t : split_ws(line)) {
+        double x;
+        if (cnt < 5 && std::sscanf(t.c_str(), "%lf", &x) == 1) b.fp[cnt++] = x;
+    }
+    if (cnt < 5) { err = "Too few FIRE parameters were supplied."; return NR_E_BATCH; }
+
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    char w1[64] = {0}, w2[64] = {0};
+    nr = std::sscanf(line.c_str(), "%d %lf %lf %d %63s %63s", &b.nmin, &b.mass, &b.fcut, &b.nmax, w1, w2);
+    if (nr < 4) { err = "One or more of nmin, mass, cutoff force and step limit were invalid."; return NR_E_BATCH; }
+    if (!std::strcmp(w1, "max") || !std::strcmp(w2, "max")) b.criterion = NR_CRIT_MAX;
+    if (!std::strcmp(w1, "cg") || !std::strcmp(w2, "cg")) b.method = NR_METHOD_CG;
+
+    double mn, mx, inc;
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
+    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of compression."; return NR_E_BATCH; }
+    b.cmin = mn; b.cmax = mx; b.cinc = inc;
+
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
+    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of shear."; return NR_E_BATCH; }
+    b.smin = mn; b.smax = mx; b.sinc = inc;
+    return NR_OK;
+}
+
+// get_file_lists, file_lists.cpp:80-137: files of `dir` matching
+// ^(<base>.*)\.dat; outputs <dir><m>_<tag>_log.txt, _edata.txt and the
+// displacement base <dir><m>_<tag>.  The four names of one network are kept
+// together (the reference sorts the four lists independently, which can pair
+// one network's input with another's outputs).
+int file_lists(const std::string &input, const std::string &tag, std::vector<JobFiles> &jobs,
+               std::string &err)
+{
+    jobs.clear();
+    if (input.empty()) { err = "empty base name"; return NR_E_BATCH; }
+    std::string dir, base;
+    size_t slash = input.find_last_of('/');
+    if (slash == std::string::npos) {
+        dir = "./";
+        base = input;
+    } else {
+        // file_lists.cpp:99-107 rebuilds the directory from '/' tokens
+        std::vector<std::string> toks;
+        std::string cur;
+        for (char c : input) {
+            if (c == '/') {
+                if (!cur.empty()) toks.push_back(cur);
+                cur.clear();
+            } else {
+                cur.push_back(c);
+            }
+        }
+        if (!cur.empty()) toks.push_back(cur);
+        base = toks.empty() ? "" : toks.back();
+        std::ostringstream oss;
+        if (input[0] == '/') oss << "/";
+        for (size_t i = 0; i + 1 < toks.size(); i++) oss << toks[i] << "/";
+        dir = oss.str();
+        if (input.back() == '/') {  // trailing slash: last token is a directory
+            oss << base << "/";
+            dir = oss.str();
+            base = "";
+        }
+    }
+    if (base == "*") base = "";
+    std::regex expr;
+    try {
+        expr = std::regex("^(" + base + ".*)\\.dat");
+    } catch (const std::regex_error &) {
+        err = "base name is not a valid pattern: " + base;
+        return NR_E_BATCH;
+    }
+    std::error_code ec;
+    std::filesystem::path p(dir);
+    if (!std::filesystem::exists(p, ec)) return NR_OK;
+    std::vector<std::string> names;
+    for (const auto &ent : std::filesystem::directory_iterator(p, ec)) {
+        if (ent.is_directory(ec)) continue;
+        names.push_back(ent.path().filename().string());
+    }
+    std::sort(names.begin(), names.end());
+    for (const std::string &name : names) {
+        std::smatch m;
+        if (!std::regex_search(name, m, expr)) continue;
+        JobFiles j;
+        std::string stem = m[1].str();
+        std::string t = tag.empty() ? "" : "_" + tag;
+        j.dat = dir + m[0].str();
+        j.log = dir + stem + t + "_log.txt";
+        j.edata = dir + stem + t + "_edata.txt";
+        j.disp = dir + stem + t;
+        jobs.push_back(j);
+    }
+    std::sort(jobs.begin(), jobs.end(), [](const JobFiles &a, const JobFiles &b) { return a.dat < b.dat; });
+    return NR_OK;
+}
+
+int dec_digits(int value)
+{
+    int d = 0;
+    do {
+        d++;
+        value /= 10;
+    } while (value > 0);
+    return d;
+}
+
+// report_name :203-213
+std::string report_name(const std::string &base, int digits, int count, const std::string &ext)
+{
+    std::ostringstream oss;
+    oss << base << "_";
+    for (int k = 0; k < digits - dec_digits(count); k++) oss << 0;
+    oss << count << "." << ext;
+    return oss.str();
+}
+
+namespace {
+
+void log_result(FILE *lf, const nr_report &r, double fcut, int criterion)
+{
+    const double measure = criterion == NR_CRIT_MAX ? r.max_force : r.rms_force;
+    if (lf) {
+        std::fprintf(lf, measure <= fcut ? "Convergence reached: " : "Convergence not reached: ");
+        std::fprintf(lf, "Steps: %d, RMS Force: %12.8le\n", r.steps, r.rms_force);
+    }
+    std::printf("Steps: %d, RMS Force: %12.8le\n", r.steps, r.rms_force);
+}
+
+}  // namespace
+
+// do_relaxation_run :792-1010.
+int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
+                 const nr_batch_opts *opts, std::string &err)
+{
+    std::clock_t ct = std::clock();
+    nr_network *net = nullptr;
+    int rc = nr_network_read(jf.dat.c_str(), &net);
+    if (rc) {
+        err = nr_last_error();
+        return rc;
+    }
+    struct Guard {
+        nr_network *n = nullptr;
+        nr_batch *b = nullptr;
+        ~Guard()
+        {
+            nr_batch_destroy(b);
+            nr_network_free(n);
+        }
+    } g;
+    g.n = net;
+    if (b.kb > 0) {
+        nr_network_build_triples(net);
+        if (b.kappa_hi > 0) nr_network_set_fiber_kappa(net, b.kappa_lo, b.kappa_hi, b.kappa_seed);
+    }
+    nr_network_info info;
+    nr_network_get_info(net, &info);
+    const double range = info.height > 0 ? info.height : info.maxy - info.miny;
+    const double smult = strain_mode ? range : 1.0;
+    if ((b.cmax > 0 && !(b.cinc > 0)) || (b.smax > 0 && !(b.sinc > 0))) {
+        err = "a strain ramp with a maximum above 0 needs a positive increment";
+        return NR_E_BATCH;
+    }
+    if (info.height > 0 && b.cmax > 0) {
+        err = "compression is not defined in the Lees-Edwards box";
+        return NR_E_BATCH;
+    }
+    const int num_digits = dec_digits(1 + (b.sinc > 0 ? (int)((b.smax - b.smin) / b.sinc) : 0));
+    rc = nr_batch_create(1, &net, &b.ks, &b.kb, opts, &g.b);
+    if (rc) {
+        err = nr_last_error();
+        return rc;
+    }
+    nr_relax_params rp;
+    rp.alpha0 = b.fp[0];
+    rp.f_alpha = b.fp[1];
+    rp.dt_max = b.fp[2];
+    rp.f_inc = b.fp[3];
+    rp.f_dec = b.fp[4];
+    rp.nmin = b.nmin;
+    rp.mass = b.mass;
+    rp.fcut = b.fcut;
+    rp.max_steps = b.nmax;
+    rp.criterion = b.criterion;
+    rp.method = b.method;
+
+    FILE *lf = std::fopen(jf.log.c_str(), "w");
+    FILE *ef = std::fopen(jf.edata.c_str(), "w");
+    std::vector<double> pos(2 * (size_t)info.n_nodes);
+    double mn = b.cmin * smult, mx = b.cmax * smult, inc = b.cinc * smult, cur = 0.0;
+    nr_report rep;
+    if (mx > 0) {
+        for (cur = mn; cur <= mx + kFltTol; cur += inc) {
+            if ((rc = nr_batch_strain(g.b, 0, 0.0, -inc)) != 0 ||
+                (rc = nr_batch_relax(g.b, &rp, 1, &rep)) != 0) {
+                err = nr_last_error();
+                break;
+            }
+            log_result(lf, rep, b.fcut, b.criterion);
+            if (ef)
+                std::fprintf(ef, "%12.0lf\t%12.10lf\t%12.8le\t%12.8le\t%12.8le\n", 0., cur,
+                             rep.e_stretch, rep.e_bend, rep.e_stretch + rep.e_bend);
+        }
+    }
+    const double max_comp = mx > mn ? cur - inc : mx;
+    mn = b.smin * smult;
+    mx = b.smax * smult;
+    inc = b.sinc * smult;
+    int disp_count = 1;
+    if (rc == 0 && mx > 0) {
+        for (cur = mn; cur <= mx + kFltTol; cur += inc) {
+            if ((rc = nr_batch_strain(g.b, 0, inc, 0.0)) != 0 ||
+                (rc = nr_batch_relax(g.b, &rp, 1, &rep)) != 0 ||
+                (rc = nr_batch_get_positions(g.b, 0, pos.data())) != 0) {
+                err = nr_last_error();
+                break;
+            }
+            log_result(lf, rep, b.fcut, b.criterion);
+            if (ef)
+                std::fprintf(ef, "%12.10lf\t%12.10lf\t%12.10le\t%12.10le\t%12.10le\n", cur, max_comp,
+                             rep.e_stretch, rep.e_bend, rep.e_stretch + rep.e_bend);
+            std::string name = report_name(jf.disp, num_digits, disp_count++, "pos");
+            FILE *pf = std::fopen(name.c_str(), "w");
+            if (pf) {
+                for (int k = 0; k < info.n_nodes; k++)
+                    std::fprintf(pf, "%2.10le\t%2.10le\n", pos[2 * k], pos[2 * k + 1]);
+                std::fclose(pf);
+            }
+        }
+    }
+    ct = std::clock() - ct;
+    if (lf) {
+        std::fprintf(lf, "Elapsed time: %f seconds\n", ((float)ct) / CLOCKS_PER_SEC);
+        std::fclose(lf);
+    }
+    if (ef) std::fclose(ef);
+    return rc;
+}
+
+}  // namespace nr
+
+extern "C" int nr_relaxation_batch(const char *batch_file, int strain_mode,
+                                   const nr_batch_opts *opts)
+{
+    std::string err;
+    nr::BatchSpec b;
+    if (!batch_file) return NR_E_ARG;
+    int rc = nr::parse_batch_file(batch_file, b, err);
+    if (rc) {
+        std::fprintf(stderr, "%s\nParsing of the batch file failed.\n", err.c_str());
+        return rc;
+    }
+    std::vector<nr::JobFiles> jobs;
+    rc = nr::file_lists(b.base, b.tag, jobs, err);
+    if (rc) {
+        std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
+        return rc;
+    }
+    int first_err = NR_OK;
+    for (const nr::JobFiles &j : jobs) {
+        rc = nr::run_protocol(j, b, strain_mode != 0, opts, err);
+        if (rc) {
+            std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
+            if (!first_err) first_err = rc;
+        }
+    }
+    return first_err;
+}

@@ -1,0 +1,39 @@
+// Batch-file protocol of the reference's `lsrelax`.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/nrelax.h"
+
+namespace nr {
+
+struct BatchSpec {
+    std::string base, tag;
+    double ks = 0, kb = 0;
+    double fp[5] = {0, 0, 0, 0, 0};
+    int nmin = 0;
+    double mass = 0, fcut = 0;
+    int nmax = 0;
+    double cmin = 0, cmax = 0, cinc = 0;
+    double smin = 0, smax = 0, sinc = 0;
+    int criterion = NR_CRIT_RMS;
+    int method = NR_METHOD_FIRE;
+    double kappa_lo = 0, kappa_hi = 0;
+    uint64_t kappa_seed = 0;
+};
+
+struct JobFiles {
+    std::string dat, log, edata, disp;
+};
+
+int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err);
+int file_lists(const std::string &input, const std::string &tag, std::vector<JobFiles> &jobs,
+               std::string &err);
+int dec_digits(int value);
+std::string report_name(const std::string &base, int digits, int count, const std::string &ext);
+int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
+                 const nr_batch_opts *opts, std::string &err);
+
+}  // namespace nr

@@ -1,0 +1,124 @@
+/*
+ * CPU FP64 oracle for the network relaxation step.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product links, loads or calls this
+ * code: only tests/, the smoke check in __graft_entry__.py and the CPU-baseline
+ * leg of bench.py may use it, and only as the checker.
+ *
+ * It restates, in plain C, the algorithm of the reference's relax step
+ * (src/large_strain_relax.cu) from the reference's behaviour, function by
+ * function (each definition in oracle.c cites the reference lines it follows):
+ *   read_network  :42-114   get_triples :116-184   clamp codes :896-902
+ *   axial_forces  :386-414  bending_forces :417-469
+ *   axial_energies :472-490 bending_energies :493-523 affine_displace :527-537
+ *   run_fire_md   :596-762  do_relaxation_run :792-1010
+ *   parse_batch_file :297-383, report_name :203-213, report_displaced :216-227
+ *   get_file_lists (src/file_lists.cpp:11-137)
+ * plus the extensions the product also carries (Lees-Edwards box, max-force
+ * criterion, per-fiber bending stiffness, nonlinear CG).
+ *
+ * Floating-point conventions shared with the product so that results can be
+ * compared tightly: axpy-style updates are fma(); squared lengths are
+ * fma(x, x, y*y); the rest length is sqrt(dx*dx + dy*dy) without contraction,
+ * as the reference's host compiler produces it; the bending angle deviation is
+ * atan2(-cross, -dot), which equals the reference's atan2(cross, dot) mapped to
+ * [0, 2pi) minus pi.
+ */
+#ifndef NR_ORACLE_H
+#define NR_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+    int n, e, t;             /* nodes, bonds, bending triples */
+    double width;            /* x period (header "offset") */
+    double height;           /* y period; 0 = reference mode (clamped rows) */
+    double le_shift;         /* Lees-Edwards image shift in x (strain*height) */
+    double miny, maxy;       /* extreme y of the input geometry */
+    double *pos;             /* 2n, input geometry */
+    int *bond;               /* 2e */
+    double *rest;            /* e */
+    signed char *bwrap;      /* 2e: wx, wy */
+    int *tri;                /* 3t: hinge, end a, end b */
+    signed char *twrap;      /* 4t: wx_a, wy_a, wx_b, wy_b */
+    double *tkb;             /* t per-triple bending stiffness, or NULL */
+    int *tbond;              /* 2t: the two bonds of each triple */
+    unsigned char *fcode;    /* n: bit0 x free, bit1 y free */
+} or_net;
+
+/* error codes */
+#define OR_OK 0
+#define OR_E_OPEN 1
+#define OR_E_HEADER 2
+#define OR_E_POINTS 3
+#define OR_E_INDEX 4
+#define OR_E_EDGES 5
+#define OR_E_ALLOC 6
+#define OR_E_ARG 7
+
+int or_read_network(const char *path, or_net *net);
+int or_net_from_arrays(int n, int e, double width, double height,
+                       const double *pos, const int *bond,
+                       const signed char *bwrap, or_net *net);
+void or_free(or_net *net);
+int or_get_triples(or_net *net);
+void or_set_clamps(or_net *net);
+int or_set_fiber_kappa(or_net *net, double klo, double khi, uint64_t seed);
+int or_fiber_labels(const or_net *net, int *label);
+
+void or_forces(const or_net *net, double ks, double kb, const double *pos,
+               double *force);
+void or_energies(const or_net *net, double ks, double kb, const double *pos,
+                 double *es, double *eb);
+void or_affine(const or_net *net, double *pos, double miny, double yrange,
+               double xdisp, double ydisp);
+
+/* FIRE parameters: alpha0, f_alpha, dt_max, f_inc, f_dec */
+#define OR_CRIT_RMS 0
+#define OR_CRIT_MAX 1
+/* report: [0] rms |F|, [1] E_stretch, [2] E_bend, [3] max node |F| */
+int or_fire(const or_net *net, double ks, double kb, double *pos,
+            const double fp[5], int nmin, double mass, double fcut,
+            int max_steps, int criterion, double report[4], int *steps);
+int or_cg(const or_net *net, double ks, double kb, double *pos, double fcut,
+          int max_iters, int criterion, double report[4], int *iters);
+
+/* Whole strain protocol with the reference's output files (strain_mode as
+ * the reference's default, absolute displacements when 0). */
+typedef struct {
+    double ks, kb;
+    double fp[5];
+    int nmin;
+    double mass, fcut;
+    int nmax;
+    double cmin, cmax, cinc;
+    double smin, smax, sinc;
+    int criterion;
+    int method;              /* 0 FIRE, 1 CG */
+    double kappa_lo, kappa_hi; /* per-fiber kappa when kappa_hi > 0 */
+    uint64_t kappa_seed;
+} or_batch;
+
+int or_parse_batch(const char *path, char *base, int base_cap, char *tag,
+                   int tag_cap, or_batch *b);
+int or_run_protocol(const char *net_file, const char *log_name,
+                    const char *edata_name, const char *disp_base,
+                    const or_batch *b, int strain_mode);
+
+int or_dec_digits(int value);
+void or_report_name(const char *base, int num_digits, int count,
+                    const char *ext, char *out, int cap);
+
+/* Synthetic triangular lattice (spacing 1, odd rows shifted by 1/2), each
+ * candidate bond (node*3 + direction) kept when u < p. */
+int or_lattice(int L, double p, uint64_t seed, int le_mode, or_net *net);
+double or_uniform(uint64_t seed, uint64_t counter);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
