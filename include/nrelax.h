@@ -82,6 +82,8 @@ typedef struct {
     int status;        /* NR_CONVERGED, NR_NOT_CONVERGED, ... */
     int fp64_steps;    /* mixed precision: steps of the FP64 polish */
     int pad;
+    double t_start;    /* device clock (s) when this network's relax began */
+    double t_done;     /* device clock (s) when it finished */
 } nr_report;
 
 typedef struct {
@@ -144,6 +146,9 @@ int nr_batch_strain(nr_batch *b, int index, double xdisp, double ydisp);
  * when shared != 0.  reports[count] receives the outcomes. */
 int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared,
                    nr_report *reports);
+/* Restores every network to its input geometry (zero strain, no pending
+ * step) without host traffic. */
+int nr_batch_reset(nr_batch *b);
 int nr_batch_get_positions(nr_batch *b, int index, double *pos);
 int nr_batch_set_positions(nr_batch *b, int index, const double *pos);
 /* One force / energy evaluation at the current positions (pending strain

@@ -40,12 +40,12 @@ class Report(C.Structure):
     _fields_ = [("rms_force", C.c_double), ("max_force", C.c_double),
                 ("e_stretch", C.c_double), ("e_bend", C.c_double),
                 ("steps", C.c_int), ("status", C.c_int), ("fp64_steps", C.c_int),
-                ("pad", C.c_int)]
+                ("pad", C.c_int), ("t_start", C.c_double), ("t_done", C.c_double)]
 
     def as_dict(self):
         return dict(rms=self.rms_force, fmax=self.max_force, es=self.e_stretch,
                     eb=self.e_bend, steps=self.steps, status=self.status,
-                    fp64_steps=self.fp64_steps)
+                    fp64_steps=self.fp64_steps, t_start=self.t_start, t_done=self.t_done)
 
 
 class BatchOpts(C.Structure):
@@ -87,6 +87,7 @@ def lib():
         L.nr_batch_create.argtypes = [C.c_int, P(vp), dp, dp, P(BatchOpts), P(vp)]
         L.nr_batch_strain.argtypes = [vp, C.c_int, C.c_double, C.c_double]
         L.nr_batch_relax.argtypes = [vp, P(RelaxParams), C.c_int, P(Report)]
+        L.nr_batch_reset.argtypes = [vp]
         L.nr_batch_get_positions.argtypes = [vp, C.c_int, dp]
         L.nr_batch_set_positions.argtypes = [vp, C.c_int, dp]
         L.nr_batch_eval.argtypes = [vp, C.c_int, dp, dp]
@@ -190,13 +191,14 @@ def relax_params(alpha0=0.1, f_alpha=0.99, dt_max=0.1, f_inc=1.1, f_dec=0.5, nmi
 class Batch:
     """Device-resident networks (nr_batch) relaxed by one persistent launch."""
 
-    def __init__(self, networks, ks, kb, precision=PREC_FP64, tile_nodes=0, team=0, device=0):
+    def __init__(self, networks, ks, kb, precision=PREC_FP64, tile_nodes=0, team=0, device=0,
+                 block=0):
         self.networks = list(networks)  # keep the host networks alive
         n = len(self.networks)
         ks = np.broadcast_to(np.asarray(ks, np.float64), (n,)).copy()
         kb = np.broadcast_to(np.asarray(kb, np.float64), (n,)).copy()
         handles = (C.c_void_p * n)(*[x.h.value for x in self.networks])
-        opts = BatchOpts(precision, 0, tile_nodes, team, device, 0)
+        opts = BatchOpts(precision, block, tile_nodes, team, device, 0)
         h = C.c_void_p()
         _check(lib().nr_batch_create(n, handles, _dp(ks), _dp(kb), C.byref(opts), C.byref(h)))
         self.h = h
@@ -227,6 +229,9 @@ class Batch:
             arr = (RelaxParams * self.count)(*params)
             _check(lib().nr_batch_relax(self.h, arr, 0, reps))
         return [r.as_dict() for r in reps]
+
+    def reset(self):
+        _check(lib().nr_batch_reset(self.h))
 
     def positions(self, index):
         out = np.zeros(2 * self.n_nodes[index])

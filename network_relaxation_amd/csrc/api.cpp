@@ -13,6 +13,7 @@
 #include "device.hpp"
 #include "network.hpp"
 #include "plan.hpp"
+#include "teams.hpp"
 
 struct nr_network {
     nr::Network net;
@@ -61,33 +62,38 @@ struct DevBuf {
     }
 };
 
+// Device tables of one persistent launch.
+struct LaunchDev {
+    nr::TeamLayout L;
+    DevBuf<nr::BlockRole> roles;
+    DevBuf<int> class_jobs, class_off, queues, mailbox;
+    DevBuf<double> partials;
+    DevBuf<unsigned long long> bar;
+};
+
 }  // namespace
 
 struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
-    int team = 1, grid = 0, npt = 1, block = 256;
-    bool xcd_local = true, cooperative = false;
+    int npt = 1, block = 256, team_cap = 256;
     size_t lds = 0;
+    std::vector<std::unique_ptr<LaunchDev>> launches;
+    std::vector<int> team_of_net;
     nr::Plan plan;
     std::vector<const nr::Network *> nets;  // borrowed for heights / miny
     std::vector<nr::NetDev> hnet;
     std::vector<double> ks, kb;
-    std::vector<int> jobs;
-    DevBuf<double2> x[2], v[2], f[2];
+    DevBuf<double2> x[2], v[2], f[2], x_init;
     DevBuf<unsigned char> mask;
-    DevBuf<int> brow, trow, halo, djobs, queue, mailbox;
-    DevBuf<uint32_t> bcode;
-    DevBuf<double> blen, tkb, partials;
-    DevBuf<uint64_t> tcode;
+    DevBuf<uint4> blob;
     DevBuf<nr::TileDev> tiles;
     DevBuf<nr::NetDev> dnet;
     DevBuf<nr::NetOut> dout;
-    DevBuf<unsigned> bar, err;
+    DevBuf<unsigned> err;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
-    int n_teams = 0;
 
     ~nr_batch()
     {
@@ -199,15 +205,19 @@ void nr_network_free(nr_network *net) { delete net; }
 
 // ---- batches --------------------------------------------------------------
 
-static int choose_shapes(nr_batch *b, const nr_batch_opts *opts)
+// Workgroup shape: `block` threads (256 or 512) each owning up to npt (1 or 2)
+// nodes of a tile; default 256 threads x 2 = 512-node tiles, two workgroups
+// per CU (LDS budget 80 KB each).
+static int choose_shapes(nr_batch *b, const nr_batch_opts *opts, int64_t *lds_budget)
 {
-    // nodes per tile: 512 by default (2 per thread of a 256-thread workgroup)
-    int tile = (opts && opts->tile_nodes > 0) ? opts->tile_nodes : 512;
-    tile = std::min(tile, 2048);
-    b->npt = 1;
-    while (b->npt * 256 < tile) b->npt *= 2;
-    if (b->npt > 8) return fail(NR_E_ARG, "tile_nodes above 2048");
+    int block = (opts && opts->block > 0) ? opts->block : 256;
+    if (block != 256 && block != 512) return fail(NR_E_ARG, "block must be 256 or 512");
+    int tile = (opts && opts->tile_nodes > 0) ? opts->tile_nodes : 2 * block;
+    if (tile > 2 * block) return fail(NR_E_ARG, "tile_nodes above 2 x block");
+    b->block = block;
+    b->npt = tile > block ? 2 : 1;
     b->plan.tile_nodes = tile;
+    *lds_budget = block == 256 ? 80 * 1024 : 156 * 1024;
     return NR_OK;
 }
 
@@ -220,7 +230,8 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     b->precision = opts ? opts->precision : NR_PREC_FP64;
     if (b->precision != NR_PREC_FP64) return fail(NR_E_ARG, "precision not available");
     HIP_TRY(hipSetDevice(b->device));
-    int rc = choose_shapes(b.get(), opts);
+    int64_t lds_budget = 0;
+    int rc = choose_shapes(b.get(), opts, &lds_budget);
     if (rc) return rc;
     for (int i = 0; i < count; i++) {
         if (!nets[i]) return fail(NR_E_ARG, "null network in batch");
@@ -230,42 +241,45 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
         b->kb.push_back(kb[i]);
     }
     std::string err;
-    rc = nr::build_plan(b->nets, b->ks, b->kb, b->plan.tile_nodes, 64 * 1024, b->plan, err);
+    rc = nr::build_plan(b->nets, b->ks, b->kb, b->plan.tile_nodes, lds_budget, b->plan, err);
     if (rc) return fail(rc, err);
     const nr::Plan &P = b->plan;
-    b->lds = (size_t)P.lds_bytes();
+    b->lds = nr::kernel_lds(P.max_blob, P.max_local, P.max_fv);
+    if (b->lds > 160 * 1024) return fail(NR_E_TOPO, "tile working set exceeds LDS");
 
-    // team shape: one network -> the whole GPU; many -> one workgroup each
+    // teams: each network gets a power-of-two team of workgroups (a function
+    // of its tile count and the cap only); classes of equal team size share
+    // the resident workgroups in proportion to their work
     int per_cu = 0, n_cu = 0;
-    HIP_TRY((hipError_t)nr::max_resident_blocks(b->npt, 256, b->lds, &per_cu, &n_cu));
+    HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
     if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
     const int resident = per_cu * n_cu;
-    int team = opts && opts->team > 0 ? opts->team : 0;
-    if (team == 0) {
-        if (count == 1) {
-            int tiles = P.nets[0].n_tiles;
-            team = std::max(1, std::min(tiles, resident));
-        } else {
-            team = 1;
+    b->team_cap = opts && opts->team > 0 ? opts->team : 256;
+    {
+        std::vector<int> tiles(count), nodes(count);
+        for (int k = 0; k < count; k++) {
+            tiles[k] = P.nets[k].n_tiles;
+            nodes[k] = P.nets[k].n_nodes;
+        }
+        std::vector<nr::TeamLayout> layouts = nr::plan_launches(tiles, nodes, b->team_cap, resident);
+        b->team_of_net.assign(count, 1);
+        for (nr::TeamLayout &L : layouts) {
+            auto ld = std::make_unique<LaunchDev>();
+            for (int k = 0; k < count; k++)
+                if (L.team_of_net[k] > 0) b->team_of_net[k] = L.team_of_net[k];
+            HIP_TRY(ld->roles.upload(L.roles));
+            HIP_TRY(ld->class_jobs.upload(L.class_jobs));
+            HIP_TRY(ld->class_off.upload(L.class_off));
+            HIP_TRY(ld->queues.alloc((size_t)32 * (L.class_off.size() - 1)));
+            HIP_TRY(ld->mailbox.alloc((size_t)32 * std::max(1, L.n_teams)));
+            HIP_TRY(ld->partials.alloc((size_t)4 * std::max(1, L.n_parts)));
+            HIP_TRY(ld->bar.alloc((size_t)nr::kBarStride * std::max(1, L.n_teams)));
+            HIP_TRY(hipMemset(ld->bar.p, 0, sizeof(unsigned long long) * nr::kBarStride *
+                                                 std::max(1, L.n_teams)));
+            ld->L = std::move(L);
+            b->launches.push_back(std::move(ld));
         }
     }
-    b->team = team;
-    if (team > 1 && count == 1) {
-        b->xcd_local = false;
-        b->grid = team;
-        b->n_teams = 1;
-    } else if (team > 1) {
-        b->xcd_local = true;
-        int per = std::max(1, resident / (8 * team));
-        b->grid = 8 * team * per;
-        b->n_teams = b->grid / team;
-    } else {
-        b->xcd_local = false;
-        b->grid = std::max(1, std::min(resident, count));
-        b->n_teams = b->grid;
-    }
-    if (b->grid > resident && team > 1) return fail(NR_E_ARG, "team exceeds resident workgroups");
-    b->cooperative = team > 1;
 
     // uploads
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -282,19 +296,18 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
         }
     }
     std::vector<double2> zeros(N, make_double2(0.0, 0.0));
+    HIP_TRY(b->x_init.upload(x0));
     for (int s = 0; s < 2; s++) {
         HIP_TRY(b->x[s].upload(x0));
         HIP_TRY(b->v[s].upload(zeros));
         HIP_TRY(b->f[s].upload(zeros));
     }
     HIP_TRY(b->mask.upload(P.mask));
-    HIP_TRY(b->brow.upload(P.brow));
-    HIP_TRY(b->trow.upload(P.trow));
-    HIP_TRY(b->halo.upload(P.halo));
-    HIP_TRY(b->bcode.upload(P.bcode));
-    HIP_TRY(b->blen.upload(P.blen));
-    HIP_TRY(b->tcode.upload(P.tcode));
-    HIP_TRY(b->tkb.upload(P.tkb));
+    {
+        std::vector<uint4> blob16(P.blob.size() / 16);
+        if (!blob16.empty()) std::memcpy(blob16.data(), P.blob.data(), P.blob.size());
+        HIP_TRY(b->blob.upload(blob16));
+    }
     HIP_TRY(b->tiles.upload(P.tiles));
     b->hnet.resize(count);
     for (int k = 0; k < count; k++) {
@@ -315,18 +328,6 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     }
     HIP_TRY(b->dnet.upload(b->hnet));
     HIP_TRY(b->dout.alloc(count));
-    // jobs: largest first (cost ~ N^1.5)
-    b->jobs.resize(count);
-    for (int k = 0; k < count; k++) b->jobs[k] = k;
-    std::stable_sort(b->jobs.begin(), b->jobs.end(), [&](int a, int c) {
-        return P.nets[a].n_nodes > P.nets[c].n_nodes;
-    });
-    HIP_TRY(b->djobs.upload(b->jobs));
-    HIP_TRY(b->queue.alloc(1));
-    HIP_TRY(b->partials.alloc((size_t)b->n_teams * b->team * 4));
-    HIP_TRY(b->bar.alloc((size_t)b->n_teams * 64));
-    HIP_TRY(hipMemset(b->bar.p, 0, sizeof(unsigned) * b->n_teams * 64));
-    HIP_TRY(b->mailbox.alloc((size_t)b->n_teams * 32));
     HIP_TRY(b->err.alloc(1));
     HIP_TRY(hipMemset(b->err.p, 0, sizeof(unsigned)));
     *out = b.release();
@@ -352,7 +353,6 @@ static int run_kernel(nr_batch *b, int mode)
     HIP_TRY(hipSetDevice(b->device));
     HIP_TRY(hipMemcpyAsync(b->dnet.p, b->hnet.data(), sizeof(nr::NetDev) * b->hnet.size(),
                            hipMemcpyHostToDevice, b->stream));
-    HIP_TRY(hipMemsetAsync(b->queue.p, 0, sizeof(int), b->stream));
     nr::KArgs a;
     std::memset(&a, 0, sizeof(a));
     for (int s = 0; s < 2; s++) {
@@ -361,34 +361,30 @@ static int run_kernel(nr_batch *b, int mode)
         a.f[s] = b->f[s].p;
     }
     a.mask = b->mask.p;
-    a.brow = b->brow.p;
-    a.trow = b->trow.p;
-    a.bcode = b->bcode.p;
-    a.blen = b->blen.p;
-    a.tcode = b->tcode.p;
-    a.tkb = b->tkb.p;
-    a.halo = b->halo.p;
+    a.blob = b->blob.p;
     a.tiles = b->tiles.p;
     a.nets = b->dnet.p;
     a.out = b->dout.p;
-    a.jobs = b->djobs.p;
-    a.n_jobs = (int)b->jobs.size();
-    a.team = b->team;
-    a.xcd_local = b->xcd_local ? 1 : 0;
-    a.has_tkb = b->plan.has_tkb ? 1 : 0;
+    a.max_blob = b->plan.max_blob;
     a.max_local = b->plan.max_local;
-    a.max_nb = b->plan.max_nb;
-    a.max_nt = b->plan.max_nt;
+    a.max_fv = b->plan.max_fv;
     a.mode = mode;
-    a.queue = b->queue.p;
-    a.partials = b->partials.p;
-    a.bar = b->bar.p;
-    a.mailbox = b->mailbox.p;
     a.err = b->err.p;
     a.timeout_ticks = 100000000ll * 60;  // 60 s at the 100 MHz constant clock
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
-    int e = nr::launch_relax(a, b->npt, b->grid, 256, b->lds, b->cooperative, b->stream);
-    if (e) return hip_fail((hipError_t)e, "relax kernel launch");
+    for (auto &ld : b->launches) {
+        HIP_TRY(hipMemsetAsync(ld->queues.p, 0, sizeof(int) * ld->queues.n, b->stream));
+        a.roles = ld->roles.p;
+        a.class_jobs = ld->class_jobs.p;
+        a.class_off = ld->class_off.p;
+        a.queues = ld->queues.p;
+        a.partials = ld->partials.p;
+        a.bar = ld->bar.p;
+        a.mailbox = ld->mailbox.p;
+        int e = nr::launch_relax(a, b->block, b->npt, ld->L.grid, b->lds, ld->L.cooperative,
+                                 b->stream);
+        if (e) return hip_fail((hipError_t)e, "relax kernel launch");
+    }
     HIP_TRY(hipEventRecord(b->ev1, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
     float ms = 0.0f;
@@ -441,6 +437,8 @@ int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared, nr_re
             reports[k].status = o[k].status;
             reports[k].fp64_steps = o[k].fp64_steps;
             reports[k].pad = 0;
+            reports[k].t_start = (double)o[k].t_start * 1e-8;
+            reports[k].t_done = (double)o[k].t_done * 1e-8;
         }
     }
     return NR_OK;
@@ -459,6 +457,22 @@ static int copy_state(nr_batch *b, int index, const DevBuf<double2> *bufs, doubl
         int o = pn.perm[d];
         out[2 * o] = tmp[d].x;
         out[2 * o + 1] = tmp[d].y;
+    }
+    return NR_OK;
+}
+
+int nr_batch_reset(nr_batch *b)
+{
+    if (!b) return fail(NR_E_ARG, "null batch");
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipMemcpyAsync(b->x[0].p, b->x_init.p, sizeof(double2) * b->x_init.n,
+                           hipMemcpyDeviceToDevice, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    for (nr::NetDev &d : b->hnet) {
+        d.parity = 0;
+        d.le_shift = 0.0;
+        d.pend_x = 0.0;
+        d.pend_y = 0.0;
     }
     return NR_OK;
 }

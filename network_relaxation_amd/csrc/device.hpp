@@ -1,10 +1,12 @@
 // Structures shared by the host driver and the gfx950 kernels.
 #pragma once
 
+#include <hip/hip_runtime.h>
+
 #include <cstdint>
 
-#include "plan.hpp"
 #include "../../include/nrelax.h"
+#include "plan.hpp"
 
 namespace nr {
 
@@ -23,41 +25,56 @@ struct NetDev {
 struct NetOut {
     double rms, fmax, es, eb;
     int steps, status, fp64_steps, pad;
+    long long t_start, t_done;  // 100 MHz device clock at job start / finish
 };
 
 enum KernelMode { kModeRelax = 0, kModeEval = 1 };
 
+// Role of one workgroup of the persistent grid (host-built table).
+struct BlockRole {
+    int team;     // team id (-1: idle)
+    int member;   // index within the team
+    int g;        // team size
+    int cls;      // job class (team-size class) the team serves
+    int part;     // first partial-sum slot of the team (4 doubles per member)
+    int pad[3];
+};
+
 struct KArgs {
     double2 *x[2], *v[2], *f[2];
     const unsigned char *mask;
-    const int *brow, *trow;
-    const uint32_t *bcode;
-    const double *blen;
-    const uint64_t *tcode;
-    const double *tkb;
-    const int *halo;
+    const uint4 *blob;
     const TileDev *tiles;
     NetDev *nets;
     NetOut *out;
-    const int *jobs;
-    int n_jobs;
-    int team;          // workgroups per network
-    int xcd_local;     // teams formed from workgroups of one XCD
-    int has_tkb;
-    int max_local, max_nb, max_nt;
+    const BlockRole *roles;   // per workgroup
+    const int *class_jobs;    // networks of each class, largest first
+    const int *class_off;     // [n_classes + 1]
+    int *queues;              // per class job counter, stride 32
+    int max_blob, max_local, max_fv;
     int mode;
-    int *queue;        // job counter
-    double *partials;  // [n_teams][team][4]
-    unsigned *bar;     // per team: [team*64] count, [team*64+32] generation
-    int *mailbox;      // per team, stride 32
-    unsigned *err;     // nonzero after a bounded wait expired
+    double *partials;         // 4 doubles per team member
+    unsigned long long *bar;  // per team kBarStride words: see team_arrive
+    int *mailbox;             // per team, stride 32
+    unsigned *err;            // nonzero after a bounded wait expired
     long long timeout_ticks;
 };
 
-// Host entry: launches the persistent relax kernel.  grid workgroups of
-// `block` threads, nodes-per-thread npt; returns a hipError_t value.
-int launch_relax(const KArgs &a, int npt, int grid, int block, size_t lds,
-                 bool cooperative, void *stream);
-int max_resident_blocks(int npt, int block, size_t lds, int *per_cu, int *n_cu);
+// Team barrier words per team: [0] root arrivals, [16] generation,
+// [32 + 16 g] arrivals of group g (groups of kBarGroup members).
+constexpr int kBarGroup = 8;
+constexpr int kBarStride = 32 + 16 * 64;
+
+// LDS bytes of the relax kernel for a plan's maxima.
+inline size_t kernel_lds(int max_blob, int max_local, int max_fv)
+{
+    return (size_t)align16(max_blob) + 16u * max_local + 16u * max_fv;
+}
+
+// Host entry: launches the persistent relax kernel (grid workgroups of
+// `block` threads, npt own nodes per thread).  Returns a hipError_t value.
+int launch_relax(const KArgs &a, int block, int npt, int grid, size_t lds, bool cooperative,
+                 void *stream);
+int max_resident_blocks(int block, int npt, size_t lds, int *per_cu, int *n_cu);
 
 }  // namespace nr

@@ -1,5 +1,7 @@
-// Device layout of a batch of networks: spatial tiles with halos and
-// per-node incidence lists (CSR) whose entries use tile-local node indices.
+// Device layout of a batch of networks: spatial tiles with halos.  Each tile's
+// topology is one 16-byte aligned blob (copied verbatim into LDS by the
+// kernel): halo node ids, bond and triple records with tile-local node
+// indices, and the node -> element incidence lists (CSR) in element order.
 #pragma once
 
 #include <cstdint>
@@ -10,26 +12,76 @@
 
 namespace nr {
 
+constexpr int kWrapBias = 8;
+
+// The element phase of the kernel writes force vectors into one LDS array:
+// [0, nb) the bond forces s r on their first node, then per triple the force
+// on end a ([nb, nb+nt)), on end b ([nb+nt, nb+2nt)) and on the hinge
+// ([nb+2nt, nb+3nt)).  An incidence entry (u16) is the index of the vector
+// its node adds, with bit 15 set when the node adds its negation (the second
+// node of a bond).
+constexpr uint16_t kIncNegate = 0x8000;
+constexpr int kMaxForceVectors = 0x7FFF;
+
 // One tile: a run of consecutive device nodes of one network (its "own"
-// nodes) plus the halo nodes its elements reach.  Local index space of a
-// tile: [0, n_own) own nodes, [n_own, n_own + n_halo) halo nodes.
+// nodes) plus the halo nodes its elements reach.  Local index space:
+// [0, n_own) own nodes, [n_own, n_own + n_halo) halo nodes.
 struct TileDev {
     int net;
-    int node0;     // first own node, global device index
-    int n_own;
-    int n_halo;
-    int halo_off;  // into the halo id array
-    int b0, nb;    // bond entries [b0, b0 + nb)
-    int t0, nt;    // triple entries [t0, t0 + nt)
+    int node0;       // first own node, global device index
+    int n_own, n_halo;
+    int nb, nt;      // bond / triple records (elements touching own nodes)
+    int n_inc;       // incidence entries
+    int blob16;      // blob offset in 16-byte units
+    int blob_bytes;  // multiple of 16
     int pad[3];
 };
 static_assert(sizeof(TileDev) == 48, "TileDev layout");
 
-// Bond entry code (u32): bits 0-15 neighbour (local), 16-19 wx+8, 20-23 wy+8,
-// bit 24 = 1 when this node is the bond's second node.
-// Triple entry code (u64): bits 0-15 hinge, 16-31 end a, 32-47 end b (local),
-// 48-51 wx_a+8, 52-55 wy_a+8, 56-59 wx_b+8, 60-63 wy_b+8.
-constexpr int kWrapBias = 8;
+// Section offsets of a tile blob, in bytes (all 16-byte aligned):
+//   halo   int32[n_halo]   global device node ids
+//   bij    u32[nb]         i | j << 16            (local indices)
+//   bw     u8[nb]          (wx + 8) | (wy + 8) << 4
+//   bl     f64[nb]         rest length
+//   tc     u64[nt]         h | a << 16 | b << 32 | wraps << 48 (4 nibbles,
+//                          wx_a, wy_a, wx_b, wy_b, each + 8)
+//   tk     f64[nt]         bending stiffness of the triple
+//   ioff   u16[n_own + 1]  incidence list start per own node
+//   inc    u16[n_inc]      force-vector index (| kIncNegate), element order
+struct BlobLayout {
+    int halo, bij, bw, bl, tc, tk, ioff, inc, bytes;
+};
+
+#ifdef __HIPCC__
+#define NR_HD __host__ __device__
+#else
+#define NR_HD
+#endif
+
+NR_HD inline int align16(int x) { return (x + 15) & ~15; }
+
+NR_HD inline BlobLayout blob_layout(int n_own, int n_halo, int nb, int nt, int n_inc)
+{
+    BlobLayout L;
+    int o = 0;
+    L.halo = o; o += (4 * n_halo + 15) & ~15;
+    L.bij = o;  o += (4 * nb + 15) & ~15;
+    L.bw = o;   o += (nb + 15) & ~15;
+    L.bl = o;   o += (8 * nb + 15) & ~15;
+    L.tc = o;   o += (8 * nt + 15) & ~15;
+    L.tk = o;   o += (8 * nt + 15) & ~15;
+    L.ioff = o; o += (2 * (n_own + 1) + 15) & ~15;
+    L.inc = o;  o += (2 * n_inc + 15) & ~15;
+    L.bytes = o;
+    return L;
+}
+
+// LDS a tile needs beyond its blob: positions (16 B per local node) and the
+// force vectors (16 B per bond, 48 B per triple).
+inline int64_t tile_work_lds(int local, int nb, int nt)
+{
+    return 16ll * local + 16ll * nb + 48ll * nt;
+}
 
 struct PlanNet {
     int node_base = 0, n_nodes = 0;
@@ -41,26 +93,15 @@ struct PlanNet {
 
 struct Plan {
     int tile_nodes = 512;
-    int lds_budget = 64 * 1024;
-    bool has_tkb = false;
-    int max_local = 0, max_nb = 0, max_nt = 0, max_own = 0;
+    int64_t lds_budget = 64 * 1024;
+    int max_local = 0, max_nb = 0, max_nt = 0, max_own = 0, max_blob = 0, max_fv = 0;
     std::vector<PlanNet> nets;
     std::vector<TileDev> tiles;
-    std::vector<int> brow, trow;           // N_total + 1, absolute entry offsets
-    std::vector<uint32_t> bcode;
-    std::vector<double> blen;
-    std::vector<uint64_t> tcode;
-    std::vector<double> tkb;               // per triple entry (has_tkb only)
-    std::vector<int> halo;
+    std::vector<uint8_t> blob;             // all tile blobs, 16-byte aligned
     std::vector<unsigned char> mask;       // device order
     int64_t n_total = 0;
 
-    // LDS bytes a tile needs in the FP64 kernel.
-    static int64_t tile_lds(int local, int nb, int nt, bool tkb)
-    {
-        return 16ll * local + 12ll * nb + (tkb ? 16ll : 8ll) * nt;
-    }
-    int64_t lds_bytes() const { return tile_lds(max_local, max_nb, max_nt, has_tkb); }
+    int64_t lds_bytes() const { return (int64_t)max_blob + 16ll * max_local + 16ll * max_fv; }
 };
 
 // Builds the plan for nets[i] with stiffness ks[i], kb[i] (elements of a
@@ -68,6 +109,6 @@ struct Plan {
 // kernels).  Returns an NR_ status.
 int build_plan(const std::vector<const Network *> &nets,
                const std::vector<double> &ks, const std::vector<double> &kb,
-               int tile_nodes, int lds_budget, Plan &plan, std::string &err);
+               int tile_nodes, int64_t lds_budget, Plan &plan, std::string &err);
 
 }  // namespace nr

@@ -2,12 +2,15 @@
 // network of a batch.  Workgroups form teams; a team takes networks from a
 // queue (largest first) and runs FIRE on each to convergence, each member
 // owning a fixed set of spatial tiles.  Per FIRE step and tile:
-//   stage the tile's bond/triple entries into LDS (coalesced copy),
-//   update own + halo positions into LDS (the deferred FIRE mix, drift and
-//   half kick of the previous step, or the affine / Lees-Edwards strain step
-//   on the first pass of a relax),
-//   gather each own node's force from its entries (no atomics; entries are in
-//   element order so the sum order is fixed), half kick, partial reductions.
+//   1. the tile's topology blob is copied into LDS with 16-byte loads (once
+//      per network when the member owns a single tile: it stays resident);
+//   2. own + halo positions are advanced into LDS: the deferred FIRE mix,
+//      drift and half kick of the previous step, or, on the first pass of a
+//      relax, the affine / Lees-Edwards strain step;
+//   3. element phase: each bond's and triple's force vectors are computed
+//      once into LDS (one atan2 per triple);
+//   4. gather: each own node adds its incident force vectors through its CSR
+//      list in element order (no atomics, fixed order), half kick, partials.
 // One team barrier per step combines the partials (P = v.F, |v|^2, |F|^2,
 // max |F_node|^2) in member order, so every member takes the same FIRE
 // decision.  References: run_fire_md large_strain_relax.cu:596-762,
@@ -23,9 +26,6 @@ namespace nr {
 
 namespace {
 
-constexpr int kBlock = 256;
-constexpr int kWaves = kBlock / 64;
-
 enum PassKind { kPassInit = 0, kPassStep = 1, kPassEnergy = 2 };
 enum MixKind { kMixNone = 0, kMixFire = 1, kMixZero = 2 };
 
@@ -34,18 +34,16 @@ struct Red4 {
 };
 
 struct StepScal {
-    double dt, dtsq, dtm;   // drift / kick of this pass
-    double alpha_old, cmix; // deferred FIRE mix of the previous step
+    double dt, dtsq, dtm;    // drift / kick of this pass
+    double alpha_old, cmix;  // deferred FIRE mix of the previous step
     int mix;
-    double pend_x, pend_y;  // affine step (init pass)
+    double pend_x, pend_y;   // affine step (init pass)
 };
 
 struct NetCtx {
     int node_base, n_nodes, tile_base, n_tiles;
     double W, H, shift, miny, yrange, ks, kb;
 };
-
-__device__ inline int wrap_of(uint32_t v) { return (int)(v & 0xF) - kWrapBias; }
 
 __device__ inline double wave_sum(double v)
 {
@@ -61,53 +59,123 @@ __device__ inline double wave_max(double v)
     return v;
 }
 
-// Bounded team barrier, executed by thread 0 of each member workgroup after
-// a __syncthreads().  Returns false when the wait bound expired.
-__device__ bool team_arrive(unsigned *cnt, unsigned *genp, unsigned g,
-                            unsigned &gen, long long timeout)
+// Data shared between the workgroups of a team (positions, velocities,
+// forces, partial sums, the job mailbox) moves through agent-scope relaxed
+// atomic loads and stores, which gfx950 performs coherently across XCDs
+// without caching stale lines.  Before a workgroup arrives at a barrier every
+// wave waits until its stores have completed, so the arrive atomic is ordered
+// after them; the barrier therefore needs no L2 write-back or invalidate.
+// A team of one workgroup uses ordinary cached accesses.
+__device__ inline double2 ld_state(bool sh, const double2 *p)
 {
-    unsigned target = gen + 1;
-    unsigned old = __hip_atomic_fetch_add(cnt, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    if (old == g - 1) {
-        __hip_atomic_store(cnt, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(genp, target, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    if (sh) {
+        unsigned long long *q = reinterpret_cast<unsigned long long *>(const_cast<double2 *>(p));
+        const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return make_double2(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
+    }
+    return *p;
+}
+
+__device__ inline void st_state(bool sh, double2 *p, double2 v)
+{
+    if (sh) {
+        unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
+        __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
     } else {
-        long long t0 = wall_clock64();
-        while (__hip_atomic_load(genp, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) != target) {
-            __builtin_amdgcn_s_sleep(2);
+        *p = v;
+    }
+}
+
+__device__ inline void st_shared_f64(double *p, double v)
+{
+    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p),
+                       (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ inline double ld_shared_f64(const double *p)
+{
+    return __longlong_as_double((long long)__hip_atomic_load(
+        reinterpret_cast<unsigned long long *>(const_cast<double *>(p)), __ATOMIC_RELAXED,
+        __HIP_MEMORY_SCOPE_AGENT));
+}
+
+// Waits until every memory operation this wave issued has completed.
+__device__ inline void drain_memory()
+{
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+}
+
+struct Team {
+    int id, member, g, part;
+    unsigned long long gen;
+};
+
+// Bounded two-level team barrier, executed by thread 0 of each member after
+// its workgroup drained its stores.  Arrival counters only grow: the member
+// that brings its group's count to a multiple of the group size arrives at
+// the root, and the one that completes the root publishes the next
+// generation.  Returns false when the wait bound expired.
+__device__ bool team_arrive(unsigned long long *bar, const Team &T, unsigned long long &gen,
+                            long long timeout)
+{
+    const unsigned long long target = gen + 1;
+    unsigned long long *root = bar, *genp = bar + 16;
+    bool last;
+    if (T.g <= kBarGroup) {
+        last = (__hip_atomic_fetch_add(root, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1) %
+                   (unsigned long long)T.g == 0;
+    } else {
+        const int grp = T.member / kBarGroup;
+        const int n_groups = (T.g + kBarGroup - 1) / kBarGroup;
+        const unsigned long long gsize = (unsigned long long)min(kBarGroup, T.g - grp * kBarGroup);
+        unsigned long long *gc = bar + 32 + 16 * grp;
+        last = (__hip_atomic_fetch_add(gc, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1) %
+                   gsize == 0;
+        if (last)
+            last = (__hip_atomic_fetch_add(root, 1ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) + 1) %
+                       (unsigned long long)n_groups == 0;
+    }
+    if (last) {
+        __hip_atomic_store(genp, target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        const long long t0 = wall_clock64();
+        while (__hip_atomic_load(genp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != target) {
+            __builtin_amdgcn_s_sleep(1);
             if (wall_clock64() - t0 > timeout) return false;
         }
     }
+    drain_memory();
     gen = target;
     return true;
 }
 
-struct Team {
-    int id, member, g;
-    unsigned gen;
-};
-
 // Whole-workgroup team barrier; all threads return the same verdict.
 __device__ bool team_sync(const KArgs &A, Team &T, int *s_flag)
 {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    drain_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
-        bool ok = team_arrive(A.bar + T.id * 64, A.bar + T.id * 64 + 32, (unsigned)T.g, T.gen,
-                              A.timeout_ticks);
+        bool ok = team_arrive(A.bar + (size_t)T.id * kBarStride, T, T.gen, A.timeout_ticks);
         if (!ok) __hip_atomic_store(A.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         *s_flag = ok ? 1 : 0;
     }
     __syncthreads();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     return *s_flag != 0;
 }
 
 // Block + team reduction of a Red4 in a fixed order (wave butterfly, waves
 // in order, members in order).  Every thread gets the result.
-__device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_red,
-                        int *s_flag)
+template <int B>
+__device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_red, int *s_flag)
 {
+    constexpr int W = B / 64;
     r.a = wave_sum(r.a);
     r.b = wave_sum(r.b);
     r.c = wave_sum(r.c);
@@ -122,57 +190,138 @@ __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_re
     __syncthreads();
     if (threadIdx.x == 0) {
         Red4 t = {0.0, 0.0, 0.0, 0.0};
-        for (int w = 0; w < kWaves; w++) {
+        for (int w = 0; w < W; w++) {
             t.a += s_red[4 * w + 0];
             t.b += s_red[4 * w + 1];
             t.c += s_red[4 * w + 2];
             t.m = fmax(t.m, s_red[4 * w + 3]);
         }
-        s_red[4 * kWaves + 0] = t.a;
-        s_red[4 * kWaves + 1] = t.b;
-        s_red[4 * kWaves + 2] = t.c;
-        s_red[4 * kWaves + 3] = t.m;
+        s_red[4 * W + 0] = t.a;
+        s_red[4 * W + 1] = t.b;
+        s_red[4 * W + 2] = t.c;
+        s_red[4 * W + 3] = t.m;
         if (T.g > 1) {
-            double *slot = A.partials + ((size_t)T.id * T.g + T.member) * 4;
-            slot[0] = t.a;
-            slot[1] = t.b;
-            slot[2] = t.c;
-            slot[3] = t.m;
+            double *slot = A.partials + ((size_t)T.part + T.member) * 4;
+            st_shared_f64(slot + 0, t.a);
+            st_shared_f64(slot + 1, t.b);
+            st_shared_f64(slot + 2, t.c);
+            st_shared_f64(slot + 3, t.m);
         }
     }
     if (T.g > 1) {
         if (!team_sync(A, T, s_flag)) return false;
-        if (threadIdx.x == 0) {
-            Red4 t = {0.0, 0.0, 0.0, 0.0};
-            const double *base = A.partials + (size_t)T.id * T.g * 4;
-            for (int m = 0; m < T.g; m++) {
-                t.a += __builtin_nontemporal_load(base + 4 * m + 0);
-                t.b += __builtin_nontemporal_load(base + 4 * m + 1);
-                t.c += __builtin_nontemporal_load(base + 4 * m + 2);
-                t.m = fmax(t.m, __builtin_nontemporal_load(base + 4 * m + 3));
+        // members' partials, one per thread, reduced by the same fixed tree
+        const double *base = A.partials + (size_t)T.part * 4;
+        Red4 t = {0.0, 0.0, 0.0, 0.0};
+        for (int m0 = 0; m0 < T.g; m0 += B) {
+            const int m = m0 + (int)threadIdx.x;
+            Red4 v = {0.0, 0.0, 0.0, 0.0};
+            if (m < T.g) {
+                v.a = ld_shared_f64(base + 4 * m + 0);
+                v.b = ld_shared_f64(base + 4 * m + 1);
+                v.c = ld_shared_f64(base + 4 * m + 2);
+                v.m = ld_shared_f64(base + 4 * m + 3);
             }
-            s_red[4 * kWaves + 0] = t.a;
-            s_red[4 * kWaves + 1] = t.b;
-            s_red[4 * kWaves + 2] = t.c;
-            s_red[4 * kWaves + 3] = t.m;
+            v.a = wave_sum(v.a);
+            v.b = wave_sum(v.b);
+            v.c = wave_sum(v.c);
+            v.m = wave_max(v.m);
+            __syncthreads();
+            if (lane == 0) {
+                s_red[4 * wave + 0] = v.a;
+                s_red[4 * wave + 1] = v.b;
+                s_red[4 * wave + 2] = v.c;
+                s_red[4 * wave + 3] = v.m;
+            }
+            __syncthreads();
+            for (int w = 0; w < W; w++) {
+                t.a += s_red[4 * w + 0];
+                t.b += s_red[4 * w + 1];
+                t.c += s_red[4 * w + 2];
+                t.m = fmax(t.m, s_red[4 * w + 3]);
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            s_red[4 * W + 0] = t.a;
+            s_red[4 * W + 1] = t.b;
+            s_red[4 * W + 2] = t.c;
+            s_red[4 * W + 3] = t.m;
         }
     }
     __syncthreads();
-    out.a = s_red[4 * kWaves + 0];
-    out.b = s_red[4 * kWaves + 1];
-    out.c = s_red[4 * kWaves + 2];
-    out.m = s_red[4 * kWaves + 3];
+    out.a = s_red[4 * W + 0];
+    out.b = s_red[4 * W + 1];
+    out.c = s_red[4 * W + 2];
+    out.m = s_red[4 * W + 3];
     __syncthreads();
     return true;
 }
 
+// LDS regions: the tile blob, then positions and force vectors.
 struct Lds {
+    unsigned char *blob;
     double2 *pos;
-    double *blen;
-    uint64_t *tcode;
-    double *tkb;
-    uint32_t *bcode;
+    double2 *fv;
 };
+
+// Views into a tile blob held in LDS.
+struct Blob {
+    const int *halo;
+    const uint32_t *bij;
+    const uint8_t *bw;
+    const double *bl;
+    const uint64_t *tc;
+    const double *tk;
+    const uint16_t *ioff;
+    const uint16_t *inc;
+};
+
+__device__ inline Blob blob_view(const unsigned char *base, const TileDev &td)
+{
+    const BlobLayout L = blob_layout(td.n_own, td.n_halo, td.nb, td.nt, td.n_inc);
+    Blob b;
+    b.halo = reinterpret_cast<const int *>(base + L.halo);
+    b.bij = reinterpret_cast<const uint32_t *>(base + L.bij);
+    b.bw = reinterpret_cast<const uint8_t *>(base + L.bw);
+    b.bl = reinterpret_cast<const double *>(base + L.bl);
+    b.tc = reinterpret_cast<const uint64_t *>(base + L.tc);
+    b.tk = reinterpret_cast<const double *>(base + L.tk);
+    b.ioff = reinterpret_cast<const uint16_t *>(base + L.ioff);
+    b.inc = reinterpret_cast<const uint16_t *>(base + L.inc);
+    return b;
+}
+
+// Copies a tile blob into LDS: 16-byte loads, four in flight per thread.
+template <int B>
+__device__ inline void stage_blob(const KArgs &A, const TileDev &td, unsigned char *dst_bytes)
+{
+    const uint4 *__restrict__ src = A.blob + td.blob16;
+    uint4 *dst = reinterpret_cast<uint4 *>(dst_bytes);
+    const int n16 = td.blob_bytes >> 4;
+    for (int base = 0; base < n16; base += 4 * B) {
+        // unconditional loads (index clamped into the blob) keep the four
+        // loads in flight together and their results in registers
+        const int i0 = base + (int)threadIdx.x, i1 = i0 + B, i2 = i0 + 2 * B, i3 = i0 + 3 * B;
+        const uint4 r0 = src[min(i0, n16 - 1)];
+        const uint4 r1 = src[min(i1, n16 - 1)];
+        const uint4 r2 = src[min(i2, n16 - 1)];
+        const uint4 r3 = src[min(i3, n16 - 1)];
+        if (i0 < n16) dst[i0] = r0;
+        if (i1 < n16) dst[i1] = r1;
+        if (i2 < n16) dst[i2] = r2;
+        if (i3 < n16) dst[i3] = r3;
+    }
+}
+
+__device__ inline int nib(uint32_t v) { return (int)(v & 0xF) - kWrapBias; }
+
+// Image offset of a neighbour with wraps (wx, wy): (wx W + wy shift, wy H).
+__device__ inline void image(const NetCtx &C, int wx, int wy, double &ox, double &oy)
+{
+    ox = C.W * wx + C.shift * wy;
+    oy = C.H * wy;
+}
 
 // New position of a node at the start of a pass.
 template <int KIND>
@@ -207,125 +356,81 @@ __device__ inline double2 advance(const double2 x, const double2 v, const double
     return xn;
 }
 
-// Force on own node k (local index) from its LDS-staged entries.
-__device__ inline double2 node_force(const Lds &L, int k, int b_lo, int b_hi, int t_lo,
-                                     int t_hi, const NetCtx &C, bool has_tkb)
+// Bond vector r = x_j + image - x_i of bond record e (axial_forces :401-404,
+// written as x_j + offset - x_i).
+__device__ inline void bond_vec(const Blob &bb, const double2 *pos, const NetCtx &C, int e,
+                                double &rx, double &ry)
+{
+    const uint32_t ij = bb.bij[e];
+    const uint32_t w = bb.bw[e];
+    const double2 pi = pos[ij & 0xFFFF], pj = pos[ij >> 16];
+    double ox, oy;
+    image(C, nib(w), nib(w >> 4), ox, oy);
+    rx = pj.x + ox - pi.x;
+    ry = pj.y + oy - pi.y;
+}
+
+// Arms of triple record t: a = x_a + img_a - x_h, b = x_b + img_b - x_h
+// (bending_forces :447-450).
+__device__ inline void tri_arms(const Blob &bb, const double2 *pos, const NetCtx &C, int t,
+                                int &h, double &ax, double &ay, double &bx, double &by)
+{
+    const uint64_t code = bb.tc[t];
+    h = (int)(code & 0xFFFF);
+    const int a = (int)((code >> 16) & 0xFFFF), c = (int)((code >> 32) & 0xFFFF);
+    const uint32_t w = (uint32_t)(code >> 48);
+    const double2 ph = pos[h], pa = pos[a], pc = pos[c];
+    double oxa, oya, oxb, oyb;
+    image(C, nib(w), nib(w >> 4), oxa, oya);
+    image(C, nib(w >> 8), nib(w >> 12), oxb, oyb);
+    ax = pa.x + oxa - ph.x;
+    ay = pa.y + oya - ph.y;
+    bx = pc.x + oxb - ph.x;
+    by = pc.y + oyb - ph.y;
+}
+
+// Force on own node k: its incident force vectors added in element order
+// (the order of a sequential pass over the bond kernel, then the triple
+// kernel).  Four entries are fetched before they are added.
+__device__ inline double2 gather_force(const Blob &bb, const double2 *fv, int k)
 {
     double fx = 0.0, fy = 0.0;
-    const double2 self = L.pos[k];
-    for (int q = b_lo; q < b_hi; q++) {
-        const uint32_t code = L.bcode[q];
-        const int nb = (int)(code & 0xFFFF);
-        const int wx = wrap_of(code >> 16), wy = wrap_of(code >> 20);
-        const double ox = C.W * wx + C.shift * wy, oy = C.H * wy;
-        const double2 other = L.pos[nb];
-        double rx, ry;
-        const bool second = (code >> 24) & 1;
-        if (!second) {
-            rx = other.x + ox - self.x;
-            ry = other.y + oy - self.y;
-        } else {
-            rx = self.x + ox - other.x;
-            ry = self.y + oy - other.y;
+    const int lo = bb.ioff[k], hi = bb.ioff[k + 1];
+    int q = lo;
+    for (; q + 4 <= hi; q += 4) {
+        double2 c[4];
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const uint32_t e = bb.inc[q + u];
+            const double2 v = fv[e & kMaxForceVectors];
+            c[u] = (e & kIncNegate) ? make_double2(-v.x, -v.y) : v;
         }
-        const double d2 = fma(rx, rx, ry * ry);
-        const double inv = 1.0 / sqrt(d2);
-        const double s = C.ks * (1.0 - L.blen[q] * inv);
-        const double bx = s * rx, by = s * ry;
-        if (!second) {
-            fx += bx;
-            fy += by;
-        } else {
-            fx += -bx;
-            fy += -by;
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            fx += c[u].x;
+            fy += c[u].y;
         }
     }
-    for (int q = t_lo; q < t_hi; q++) {
-        const uint64_t code = L.tcode[q];
-        const int h = (int)(code & 0xFFFF), a = (int)((code >> 16) & 0xFFFF),
-                  c = (int)((code >> 32) & 0xFFFF);
-        const int wxa = wrap_of((uint32_t)(code >> 48)), wya = wrap_of((uint32_t)(code >> 52));
-        const int wxb = wrap_of((uint32_t)(code >> 56)), wyb = wrap_of((uint32_t)(code >> 60));
-        const double2 ph = L.pos[h], pa = L.pos[a], pc = L.pos[c];
-        const double oxa = C.W * wxa + C.shift * wya, oya = C.H * wya;
-        const double oxb = C.W * wxb + C.shift * wyb, oyb = C.H * wyb;
-        const double ax = pa.x + oxa - ph.x, ay = pa.y + oya - ph.y;
-        const double bx = pc.x + oxb - ph.x, by = pc.y + oyb - ph.y;
-        const double ia = 1.0 / fma(ax, ax, ay * ay);
-        const double ib = 1.0 / fma(bx, bx, by * by);
-        const double cr = fma(ax, by, -(ay * bx));
-        const double dt = fma(ax, bx, ay * by);
-        // theta - pi of :450-453 as one atan2 (no cancellation near pi)
-        const double dth = atan2(-cr, -dt);
-        const double kk = has_tkb ? L.tkb[q] : C.kb;
-        const double g = kk * dth;
-        double f2x = g * ia;
-        const double f2y = f2x * ax;
-        f2x = f2x * -ay;
-        double f3x = g * ib;
-        const double f3y = -f3x * bx;
-        f3x = f3x * by;
-        if (k == h) {
-            fx += -f2x - f3x;
-            fy += -f2y - f3y;
-        }
-        if (k == a) {
-            fx += f2x;
-            fy += f2y;
-        }
-        if (k == c) {
-            fx += f3x;
-            fy += f3y;
+    for (; q < hi; q++) {
+        const uint32_t e = bb.inc[q];
+        const double2 v = fv[e & kMaxForceVectors];
+        if (e & kIncNegate) {
+            fx += -v.x;
+            fy += -v.y;
+        } else {
+            fx += v.x;
+            fy += v.y;
         }
     }
     return make_double2(fx, fy);
 }
 
-// Energy of the elements own node k is responsible for: bonds where it is
-// the first node, triples where it is the hinge.
-__device__ inline void node_energy(const Lds &L, int k, int b_lo, int b_hi, int t_lo,
-                                   int t_hi, const NetCtx &C, bool has_tkb, double &es,
-                                   double &eb)
-{
-    const double2 self = L.pos[k];
-    for (int q = b_lo; q < b_hi; q++) {
-        const uint32_t code = L.bcode[q];
-        if ((code >> 24) & 1) continue;
-        const int nb = (int)(code & 0xFFFF);
-        const int wx = wrap_of(code >> 16), wy = wrap_of(code >> 20);
-        const double ox = C.W * wx + C.shift * wy, oy = C.H * wy;
-        const double2 other = L.pos[nb];
-        const double rx = other.x + ox - self.x, ry = other.y + oy - self.y;
-        const double d = sqrt(fma(rx, rx, ry * ry));
-        const double dl = d - L.blen[q];
-        es += 0.5 * C.ks * dl * dl;
-    }
-    for (int q = t_lo; q < t_hi; q++) {
-        const uint64_t code = L.tcode[q];
-        const int h = (int)(code & 0xFFFF);
-        if (h != k) continue;
-        const int a = (int)((code >> 16) & 0xFFFF), c = (int)((code >> 32) & 0xFFFF);
-        const int wxa = wrap_of((uint32_t)(code >> 48)), wya = wrap_of((uint32_t)(code >> 52));
-        const int wxb = wrap_of((uint32_t)(code >> 56)), wyb = wrap_of((uint32_t)(code >> 60));
-        const double2 pa = L.pos[a], pc = L.pos[c];
-        const double oxa = C.W * wxa + C.shift * wya, oya = C.H * wya;
-        const double oxb = C.W * wxb + C.shift * wyb, oyb = C.H * wyb;
-        const double ax = pa.x + oxa - self.x, ay = pa.y + oya - self.y;
-        const double bx = pc.x + oxb - self.x, by = pc.y + oyb - self.y;
-        const double cr = fma(ax, by, -(ay * bx));
-        const double dt = fma(ax, bx, ay * by);
-        const double dth = atan2(-cr, -dt);
-        const double kk = has_tkb ? L.tkb[q] : C.kb;
-        eb += 0.5 * kk * dth * dth;
-    }
-}
-
 // One pass over this member's tiles.  Init/step passes write x, v, F into
 // buffer q = 1 - p and return (P, |v|^2, |F|^2, max |F_node|^2) partials;
 // the energy pass reads buffer p and returns (E_stretch, E_bend, 0, 0).
-template <int NPT, int KIND>
-__device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepScal &S,
-                     int p, const Lds &L)
+template <int B, int NPT, int KIND>
+__device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepScal &S, int p,
+                     const Lds &L, bool resident)
 {
     Red4 acc = {0.0, 0.0, 0.0, 0.0};
     const int tid = threadIdx.x;
@@ -333,73 +438,115 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     const double2 *__restrict__ X = A.x[p];
     const double2 *__restrict__ V = A.v[p];
     const double2 *__restrict__ F = A.f[p];
-    const bool has_tkb = A.has_tkb != 0;
+    const double2 zero = make_double2(0.0, 0.0);
+    const bool sh = T.g > 1;
     for (int ti = T.member; ti < C.n_tiles; ti += T.g) {
         const TileDev td = A.tiles[C.tile_base + ti];
-        // stage entries (coalesced) into LDS
-        for (int i = tid; i < td.nb; i += kBlock) {
-            L.bcode[i] = A.bcode[td.b0 + i];
-            L.blen[i] = A.blen[td.b0 + i];
+        if (!resident) {
+            stage_blob<B>(A, td, L.blob);
+            __syncthreads();
         }
-        for (int i = tid; i < td.nt; i += kBlock) {
-            L.tcode[i] = A.tcode[td.t0 + i];
-            if (has_tkb) L.tkb[i] = A.tkb[td.t0 + i];
-        }
-        // positions: own nodes keep their half-kicked velocity in registers
+        const Blob bb = blob_view(L.blob, td);
+        // positions: own nodes keep their half-kicked velocity and clamp mask
+        // in registers
         double2 vh[NPT];
+        unsigned char mk[NPT];
 #pragma unroll
         for (int j = 0; j < NPT; j++) {
-            const int k = tid + j * kBlock;
-            vh[j] = make_double2(0.0, 0.0);
+            const int k = tid + j * B;
+            vh[j] = zero;
+            mk[j] = 0;
             if (k < td.n_own) {
                 const int gid = td.node0 + k;
                 double2 xn;
                 if (KIND == kPassEnergy) {
-                    xn = X[gid];
+                    xn = ld_state(sh, X + gid);
                 } else if (KIND == kPassInit) {
-                    xn = advance<kPassInit>(X[gid], make_double2(0.0, 0.0), make_double2(0.0, 0.0),
-                                            S, C, vh[j]);
-                    A.x[q][gid] = xn;
+                    xn = advance<kPassInit>(ld_state(sh, X + gid), zero, zero, S, C, vh[j]);
+                    st_state(sh, A.x[q] + gid, xn);
                 } else {
-                    xn = advance<kPassStep>(X[gid], V[gid], F[gid], S, C, vh[j]);
-                    A.x[q][gid] = xn;
+                    xn = advance<kPassStep>(ld_state(sh, X + gid), ld_state(sh, V + gid),
+                                            ld_state(sh, F + gid), S, C, vh[j]);
+                    st_state(sh, A.x[q] + gid, xn);
                 }
+                if (KIND != kPassEnergy) mk[j] = A.mask[gid];
                 L.pos[k] = xn;
             }
         }
-        for (int k = td.n_own + tid; k < td.n_own + td.n_halo; k += kBlock) {
-            const int gid = A.halo[td.halo_off + k - td.n_own];
+        for (int k = td.n_own + tid; k < td.n_own + td.n_halo; k += B) {
+            const int gid = bb.halo[k - td.n_own];
             double2 dummy, xn;
             if (KIND == kPassEnergy)
-                xn = X[gid];
+                xn = ld_state(sh, X + gid);
             else if (KIND == kPassInit)
-                xn = advance<kPassInit>(X[gid], make_double2(0.0, 0.0), make_double2(0.0, 0.0), S,
-                                        C, dummy);
+                xn = advance<kPassInit>(ld_state(sh, X + gid), zero, zero, S, C, dummy);
             else
-                xn = advance<kPassStep>(X[gid], V[gid], F[gid], S, C, dummy);
+                xn = advance<kPassStep>(ld_state(sh, X + gid), ld_state(sh, V + gid),
+                                        ld_state(sh, F + gid), S, C, dummy);
             L.pos[k] = xn;
         }
         __syncthreads();
+        // element phase
+        for (int e = tid; e < td.nb; e += B) {
+            double rx, ry;
+            bond_vec(bb, L.pos, C, e, rx, ry);
+            if (KIND == kPassEnergy) {
+                if ((int)(bb.bij[e] & 0xFFFF) < td.n_own) {
+                    const double d = sqrt(fma(rx, rx, ry * ry));
+                    const double dl = d - bb.bl[e];
+                    acc.a += 0.5 * C.ks * dl * dl;
+                }
+            } else {
+                // :401-404: f = ks (1 - L/|r|) r on the first node
+                const double d2 = fma(rx, rx, ry * ry);
+                const double inv = 1.0 / sqrt(d2);
+                const double s = C.ks * (1.0 - bb.bl[e] * inv);
+                L.fv[e] = make_double2(s * rx, s * ry);
+            }
+        }
+        for (int t = tid; t < td.nt; t += B) {
+            int h;
+            double ax, ay, bx, by;
+            tri_arms(bb, L.pos, C, t, h, ax, ay, bx, by);
+            const double cr = fma(ax, by, -(ay * bx));
+            const double dt = fma(ax, bx, ay * by);
+            // theta - pi of :450-453 as one atan2 (no cancellation near pi)
+            const double dth = atan2(-cr, -dt);
+            const double kk = bb.tk[t];
+            if (KIND == kPassEnergy) {
+                if (h < td.n_own) acc.b += 0.5 * kk * dth * dth;
+            } else {
+                // :453-468 force on end a, end b and the hinge
+                const double ia = 1.0 / fma(ax, ax, ay * ay);
+                const double ib = 1.0 / fma(bx, bx, by * by);
+                const double g = kk * dth;
+                double f2x = g * ia;
+                const double f2y = f2x * ax;
+                f2x = f2x * -ay;
+                double f3x = g * ib;
+                const double f3y = -f3x * bx;
+                f3x = f3x * by;
+                L.fv[td.nb + t] = make_double2(f2x, f2y);
+                L.fv[td.nb + td.nt + t] = make_double2(f3x, f3y);
+                L.fv[td.nb + 2 * td.nt + t] = make_double2(-f2x - f3x, -f2y - f3y);
+            }
+        }
+        if (KIND != kPassEnergy) {
+            __syncthreads();
 #pragma unroll
-        for (int j = 0; j < NPT; j++) {
-            const int k = tid + j * kBlock;
-            if (k < td.n_own) {
-                const int gid = td.node0 + k;
-                const int b_lo = A.brow[gid] - td.b0, b_hi = A.brow[gid + 1] - td.b0;
-                const int t_lo = A.trow[gid] - td.t0, t_hi = A.trow[gid + 1] - td.t0;
-                if (KIND == kPassEnergy) {
-                    node_energy(L, k, b_lo, b_hi, t_lo, t_hi, C, has_tkb, acc.a, acc.b);
-                } else {
-                    double2 fn = node_force(L, k, b_lo, b_hi, t_lo, t_hi, C, has_tkb);
-                    const unsigned char m = A.mask[gid];
-                    if (!(m & 1)) fn.x = 0.0;
-                    if (!(m & 2)) fn.y = 0.0;
+            for (int j = 0; j < NPT; j++) {
+                const int k = tid + j * B;
+                if (k < td.n_own) {
+                    const int gid = td.node0 + k;
+                    double2 fn = gather_force(bb, L.fv, k);
+                    if (!(mk[j] & 1)) fn.x = 0.0;
+                    if (!(mk[j] & 2)) fn.y = 0.0;
                     double2 vn;
                     vn.x = fma(S.dtm, fn.x, vh[j].x);  // :677 second half kick
                     vn.y = fma(S.dtm, fn.y, vh[j].y);
-                    if (KIND == kPassInit) vn = make_double2(0.0, 0.0);
-                    A.f[q][gid] = fn;
-                    A.v[q][gid] = vn;
+                    if (KIND == kPassInit) vn = zero;
+                    st_state(sh, A.f[q] + gid, fn);
+                    st_state(sh, A.v[q] + gid, vn);
                     acc.a = fma(vn.x, fn.x, acc.a);
                     acc.a = fma(vn.y, fn.y, acc.a);
                     acc.b = fma(vn.x, vn.x, acc.b);
@@ -415,11 +562,12 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     return acc;
 }
 
-template <int NPT>
-__device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
-                              double *s_red, int *s_flag)
+template <int B, int NPT>
+__device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L, double *s_red,
+                              int *s_flag)
 {
     NetDev *nd = A.nets + net_id;
+    const long long t_start = wall_clock64();
     NetCtx C;
     C.node_base = nd->node_base;
     C.n_nodes = nd->n_nodes;
@@ -435,10 +583,17 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     // the strain step advances the Lees-Edwards image shift before the pass
     C.shift = nd->le_shift + (C.H > 0 ? pend_x : 0.0);
     int p = nd->parity;
-    const double a0 = nd->a0, fa = nd->fa, dtmax = nd->dtmax, finc = nd->finc,
-                 fdec = nd->fdec, mass = nd->mass, fcut = nd->fcut;
+    const double a0 = nd->a0, fa = nd->fa, dtmax = nd->dtmax, finc = nd->finc, fdec = nd->fdec,
+                 mass = nd->mass, fcut = nd->fcut;
     const int nmin = nd->nmin, max_steps = A.mode == kModeEval ? 0 : nd->max_steps;
     const int crit = nd->crit;
+
+    // a member owning exactly one tile keeps that tile's blob in LDS
+    const bool resident = T.member < C.n_tiles && C.n_tiles <= T.member + T.g;
+    if (resident) {
+        stage_blob<B>(A, A.tiles[C.tile_base + T.member], L.blob);
+        __syncthreads();
+    }
 
     StepScal S;
     S.dt = 0.0;
@@ -451,9 +606,9 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     S.pend_y = pend_y;
 
     int status = NR_NOT_CONVERGED;
-    Red4 r;
-    Red4 part = pass<NPT, kPassInit>(A, T, C, S, p, L);
-    if (!reduce4(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+    Red4 r = {0.0, 0.0, 0.0, 0.0};
+    Red4 part = pass<B, NPT, kPassInit>(A, T, C, S, p, L, resident);
+    if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
     p ^= 1;
     double ff = r.c, fmax2 = r.m;
     double fmag = sqrt(ff);
@@ -463,14 +618,13 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     double alpha = a0, dt = dtmax;
     double dtm = 0.5 * dt / mass, dtsq = 0.5 * dt * dt / mass;
     int since = 0, step = 0;
-    S.mix = kMixNone;
     while (status != NR_ABORTED && step < max_steps) {
         step++;
         S.dt = dt;
         S.dtsq = dtsq;
         S.dtm = dtm;
-        part = pass<NPT, kPassStep>(A, T, C, S, p, L);
-        if (!reduce4(A, T, part, r, s_red, s_flag)) {
+        part = pass<B, NPT, kPassStep>(A, T, C, S, p, L, resident);
+        if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
             status = NR_ABORTED;
             break;
         }
@@ -491,7 +645,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 alpha *= fa;
             }
             S.mix = kMixFire;
-        } else {      // :704-712
+        } else {  // :704-712
             since = 0;
             dt *= fdec;
             dtm = 0.5 * dt / mass;
@@ -499,22 +653,23 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             alpha = a0;
             S.mix = kMixZero;
         }
-        const bool conv = crit == 1 ? sqrt(fmax2) < fcut : fmag / sqrt_dof < fcut;
+        const bool conv = crit == NR_CRIT_MAX ? sqrt(fmax2) < fcut : fmag / sqrt_dof < fcut;
         if (conv) break;
         if (!isfinite(ff)) break;
     }
     if (status != NR_ABORTED) {
-        if (!isfinite(ff)) status = NR_NONFINITE;
-        else {
-            const double meas = crit == 1 ? sqrt(fmax2) : fmag / sqrt_dof;
+        if (!isfinite(ff)) {
+            status = NR_NONFINITE;
+        } else {
+            const double meas = crit == NR_CRIT_MAX ? sqrt(fmax2) : fmag / sqrt_dof;
             status = meas < fcut ? NR_CONVERGED : NR_NOT_CONVERGED;
         }
     }
     // energies at the final positions (:723-757)
     Red4 e = {0.0, 0.0, 0.0, 0.0};
     if (status != NR_ABORTED) {
-        part = pass<NPT, kPassEnergy>(A, T, C, S, p, L);
-        if (!reduce4(A, T, part, e, s_red, s_flag)) status = NR_ABORTED;
+        part = pass<B, NPT, kPassEnergy>(A, T, C, S, p, L, resident);
+        if (!reduce4<B>(A, T, part, e, s_red, s_flag)) status = NR_ABORTED;
     }
     if (T.member == 0 && threadIdx.x == 0) {
         NetOut o;
@@ -526,6 +681,8 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
         o.status = status;
         o.fp64_steps = step;
         o.pad = 0;
+        o.t_start = t_start;
+        o.t_done = wall_clock64();
         A.out[net_id] = o;
         nd->parity = p;
         nd->le_shift = C.shift;
@@ -534,37 +691,36 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     }
 }
 
-template <int NPT>
-__global__ __launch_bounds__(kBlock) void relax_kernel(KArgs A)
+template <int B, int NPT>
+__global__ __launch_bounds__(B) void relax_kernel(KArgs A)
 {
     extern __shared__ __align__(16) unsigned char smem[];
-    __shared__ double s_red[4 * (kWaves + 1)];
+    __shared__ double s_red[4 * (B / 64 + 1)];
     __shared__ int s_flag;
     __shared__ int s_job;
     Lds L;
-    L.pos = reinterpret_cast<double2 *>(smem);
-    L.blen = reinterpret_cast<double *>(L.pos + A.max_local);
-    L.tcode = reinterpret_cast<uint64_t *>(L.blen + A.max_nb);
-    L.tkb = reinterpret_cast<double *>(L.tcode + A.max_nt);
-    L.bcode = reinterpret_cast<uint32_t *>(L.tkb + (A.has_tkb ? A.max_nt : 0));
+    L.blob = smem;
+    L.pos = reinterpret_cast<double2 *>(smem + align16(A.max_blob));
+    L.fv = L.pos + A.max_local;
 
+    const BlockRole role = A.roles[blockIdx.x];
+    if (role.team < 0) return;  // idle workgroup
     Team T;
-    T.g = A.team;
-    if (A.xcd_local) {
-        const int xcd = blockIdx.x & 7, slot = blockIdx.x >> 3;
-        T.id = xcd + 8 * (slot / T.g);
-        T.member = slot % T.g;
-    } else {
-        T.id = blockIdx.x / T.g;
-        T.member = blockIdx.x % T.g;
-    }
+    T.id = role.team;
+    T.member = role.member;
+    T.g = role.g;
+    T.part = role.part;
     T.gen = 0;
     if (T.g > 1 && threadIdx.x == 0)
-        T.gen = __hip_atomic_load(A.bar + T.id * 64 + 32, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        T.gen = __hip_atomic_load(A.bar + (size_t)T.id * kBarStride + 16, __ATOMIC_RELAXED,
+                                  __HIP_MEMORY_SCOPE_AGENT);
+    int *queue = A.queues + 32 * role.cls;
+    const int *jobs = A.class_jobs + A.class_off[role.cls];
+    const int n_jobs = A.class_off[role.cls + 1] - A.class_off[role.cls];
 
     for (;;) {
         if (threadIdx.x == 0 && T.member == 0) {
-            const int j = __hip_atomic_fetch_add(A.queue, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            const int j = __hip_atomic_fetch_add(queue, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_job = j;
             if (T.g > 1)
                 __hip_atomic_store(A.mailbox + T.id * 32, j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -577,28 +733,30 @@ __global__ __launch_bounds__(kBlock) void relax_kernel(KArgs A)
         __syncthreads();
         const int job = s_job;
         __syncthreads();
-        if (job >= A.n_jobs) return;
-        relax_network<NPT>(A, T, A.jobs[job], L, s_red, &s_flag);
+        if (job >= n_jobs) return;
+        relax_network<B, NPT>(A, T, jobs[job], L, s_red, &s_flag);
         if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     }
 }
 
+const void *kernel_for(int block, int npt)
+{
+    if (block == 256 && npt == 1) return (const void *)relax_kernel<256, 1>;
+    if (block == 256 && npt == 2) return (const void *)relax_kernel<256, 2>;
+    if (block == 512 && npt == 1) return (const void *)relax_kernel<512, 1>;
+    if (block == 512 && npt == 2) return (const void *)relax_kernel<512, 2>;
+    return nullptr;
+}
+
 }  // namespace
 
-int launch_relax(const KArgs &a, int npt, int grid, int block, size_t lds, bool cooperative,
+int launch_relax(const KArgs &a, int block, int npt, int grid, size_t lds, bool cooperative,
                  void *stream)
 {
-    if (block != kBlock) return (int)hipErrorInvalidValue;
+    const void *fn = kernel_for(block, npt);
+    if (!fn) return (int)hipErrorInvalidValue;
     hipStream_t s = (hipStream_t)stream;
     void *args[] = {const_cast<KArgs *>(&a)};
-    const void *fn = nullptr;
-    switch (npt) {
-    case 1: fn = (const void *)relax_kernel<1>; break;
-    case 2: fn = (const void *)relax_kernel<2>; break;
-    case 4: fn = (const void *)relax_kernel<4>; break;
-    case 8: fn = (const void *)relax_kernel<8>; break;
-    default: return (int)hipErrorInvalidValue;
-    }
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
     if (cooperative)
@@ -608,16 +766,10 @@ int launch_relax(const KArgs &a, int npt, int grid, int block, size_t lds, bool 
     return (int)e;
 }
 
-int max_resident_blocks(int npt, int block, size_t lds, int *per_cu, int *n_cu)
+int max_resident_blocks(int block, int npt, size_t lds, int *per_cu, int *n_cu)
 {
-    const void *fn = nullptr;
-    switch (npt) {
-    case 1: fn = (const void *)relax_kernel<1>; break;
-    case 2: fn = (const void *)relax_kernel<2>; break;
-    case 4: fn = (const void *)relax_kernel<4>; break;
-    case 8: fn = (const void *)relax_kernel<8>; break;
-    default: return (int)hipErrorInvalidValue;
-    }
+    const void *fn = kernel_for(block, npt);
+    if (!fn) return (int)hipErrorInvalidValue;
     hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
     if (e != hipSuccess) return (int)e;
     int nb = 0;
