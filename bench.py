@@ -67,12 +67,18 @@ def main():
     ap.add_argument("--max-steps", type=int, default=3000000)
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--team", type=int, default=0)
+    ap.add_argument("--tpm", type=int, default=0, help="tiles per team member")
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
+                    help="only run a device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: the batched relax shards by network "
@@ -83,7 +89,8 @@ def main():
 
     specs = workload(args.networks, args.lmin, args.lmax, args.seed)
     nets = build_networks(N, specs)
-    batch = N.Batch(nets, 1.0, 1.0, tile_nodes=args.tile, team=args.team)
+    batch = N.Batch(nets, 1.0, 1.0, tile_nodes=args.tile, team=args.team,
+                    tiles_per_member=args.tpm)
     prm = N.relax_params(fcut=args.fcut, max_steps=args.max_steps, criterion=N.CRIT_MAX, **FIRE)
 
     def one_step():
@@ -125,7 +132,7 @@ def main():
         t_plain = 0.0
         ok = 0
         for k in idx:
-            b1 = N.Batch([nets[k]], 1.0, 1.0, team=0)
+            b1 = N.Batch([nets[k]], 1.0, 1.0, tiles_per_member=1)
             b1.shear(0, specs[k]["gamma"])
             ts = time.perf_counter()
             r1 = b1.relax(prm)[0]
@@ -172,6 +179,19 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def calibrate_copy(mib):
+    """A plain device-to-device copy kernel of known size (torch's elementwise
+    copy), so that the FETCH_SIZE / WRITE_SIZE counters can be scaled."""
+    import torch
+    n = mib * (1 << 20) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration_copy_bytes": 4 * n, "launches": 3}), flush=True)
 
 
 def cpu_baseline(specs, reps, args):

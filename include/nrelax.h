@@ -90,9 +90,10 @@ typedef struct {
     int precision;     /* NR_PREC_FP64 / NR_PREC_MIXED */
     int block;         /* threads per workgroup, 0 = tuned */
     int tile_nodes;    /* nodes per tile, 0 = tuned */
-    int team;          /* workgroups per network, 0 = tuned */
+    int team;          /* most workgroups per network, 0 = tuned */
     int device;        /* HIP device ordinal */
-    int pad;
+    int tiles_per_member; /* a network of T tiles gets a team of about
+                             T / tiles_per_member workgroups, 0 = tuned */
 } nr_batch_opts;
 
 /* ---- errors ------------------------------------------------------------ */

@@ -76,7 +76,7 @@ struct LaunchDev {
 struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
-    int npt = 1, block = 256, team_cap = 256;
+    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 8;
     size_t lds = 0;
     std::vector<std::unique_ptr<LaunchDev>> launches;
     std::vector<int> team_of_net;
@@ -244,7 +244,7 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     rc = nr::build_plan(b->nets, b->ks, b->kb, b->plan.tile_nodes, lds_budget, b->plan, err);
     if (rc) return fail(rc, err);
     const nr::Plan &P = b->plan;
-    b->lds = nr::kernel_lds(P.max_blob, P.max_local, P.max_fv);
+    b->lds = (size_t)P.lds_bytes();
     if (b->lds > 160 * 1024) return fail(NR_E_TOPO, "tile working set exceeds LDS");
 
     // teams: each network gets a power-of-two team of workgroups (a function
@@ -255,13 +255,15 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
     const int resident = per_cu * n_cu;
     b->team_cap = opts && opts->team > 0 ? opts->team : 256;
+    b->tiles_per_member = opts && opts->tiles_per_member > 0 ? opts->tiles_per_member : 8;
+    if (b->team_cap > resident) b->team_cap = resident;
     {
-        std::vector<int> tiles(count), nodes(count);
+        std::vector<int> team(count), nodes(count);
         for (int k = 0; k < count; k++) {
-            tiles[k] = P.nets[k].n_tiles;
+            team[k] = nr::team_size(P.nets[k].n_tiles, b->team_cap, b->tiles_per_member);
             nodes[k] = P.nets[k].n_nodes;
         }
-        std::vector<nr::TeamLayout> layouts = nr::plan_launches(tiles, nodes, b->team_cap, resident);
+        std::vector<nr::TeamLayout> layouts = nr::plan_launches(team, nodes, resident);
         b->team_of_net.assign(count, 1);
         for (nr::TeamLayout &L : layouts) {
             auto ld = std::make_unique<LaunchDev>();

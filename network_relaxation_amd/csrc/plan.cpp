@@ -283,6 +283,7 @@ int build_plan(const std::vector<const Network *> &nets,
             plan.max_nt = std::max(plan.max_nt, nt);
             plan.max_blob = std::max(plan.max_blob, Lb.bytes);
             plan.max_fv = std::max(plan.max_fv, nb + 3 * nt);
+            plan.max_tile_lds = std::max(plan.max_tile_lds, need);
             plan.tiles.push_back(td);
             clear();
         }

@@ -95,13 +95,14 @@ struct Plan {
     int tile_nodes = 512;
     int64_t lds_budget = 64 * 1024;
     int max_local = 0, max_nb = 0, max_nt = 0, max_own = 0, max_blob = 0, max_fv = 0;
+    int64_t max_tile_lds = 0;  // largest blob + positions + force vectors of a tile
     std::vector<PlanNet> nets;
     std::vector<TileDev> tiles;
     std::vector<uint8_t> blob;             // all tile blobs, 16-byte aligned
     std::vector<unsigned char> mask;       // device order
     int64_t n_total = 0;
 
-    int64_t lds_bytes() const { return (int64_t)max_blob + 16ll * max_local + 16ll * max_fv; }
+    int64_t lds_bytes() const { return max_tile_lds; }
 };
 
 // Builds the plan for nets[i] with stiffness ks[i], kb[i] (elements of a

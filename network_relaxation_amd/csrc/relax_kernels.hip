@@ -430,7 +430,7 @@ __device__ inline double2 gather_force(const Blob &bb, const double2 *fv, int k)
 // the energy pass reads buffer p and returns (E_stretch, E_bend, 0, 0).
 template <int B, int NPT, int KIND>
 __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepScal &S, int p,
-                     const Lds &L, bool resident)
+                     const Lds &L0, bool resident)
 {
     Red4 acc = {0.0, 0.0, 0.0, 0.0};
     const int tid = threadIdx.x;
@@ -442,6 +442,10 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     const bool sh = T.g > 1;
     for (int ti = T.member; ti < C.n_tiles; ti += T.g) {
         const TileDev td = A.tiles[C.tile_base + ti];
+        // positions and force vectors follow this tile's blob
+        Lds L = L0;
+        L.pos = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
+        L.fv = L.pos + td.n_own + td.n_halo;
         if (!resident) {
             stage_blob<B>(A, td, L.blob);
             __syncthreads();
@@ -700,8 +704,8 @@ __global__ __launch_bounds__(B) void relax_kernel(KArgs A)
     __shared__ int s_job;
     Lds L;
     L.blob = smem;
-    L.pos = reinterpret_cast<double2 *>(smem + align16(A.max_blob));
-    L.fv = L.pos + A.max_local;
+    L.pos = nullptr;  // placed per tile, after its blob
+    L.fv = nullptr;
 
     const BlockRole role = A.roles[blockIdx.x];
     if (role.team < 0) return;  // idle workgroup

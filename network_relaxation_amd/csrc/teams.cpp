@@ -9,25 +9,23 @@
 
 namespace nr {
 
-int team_size(int n_tiles, int cap)
+int team_size(int n_tiles, int cap, int tiles_per_member)
 {
-    int lim = std::max(1, std::min(n_tiles, cap));
+    const int r = std::max(1, tiles_per_member);
+    int lim = std::max(1, std::min((n_tiles + r - 1) / r, cap));
     int g = 1;
     while (2 * g <= lim) g *= 2;
     return g;
 }
 
-TeamLayout plan_teams(const std::vector<int> &n_tiles, const std::vector<int> &n_nodes,
-                      int cap, int resident)
+TeamLayout plan_teams(const std::vector<int> &team, const std::vector<int> &n_nodes,
+                      int resident)
 {
     TeamLayout T;
-    const int count = (int)n_tiles.size();
-    T.team_of_net.resize(count);
+    const int count = (int)team.size();
+    T.team_of_net = team;
     std::map<int, std::vector<int>> classes;  // team size -> networks
-    for (int k = 0; k < count; k++) {
-        T.team_of_net[k] = team_size(n_tiles[k], std::min(cap, resident));
-        classes[T.team_of_net[k]].push_back(k);
-    }
+    for (int k = 0; k < count; k++) classes[team[k]].push_back(k);
     std::vector<int> gs;
     std::vector<double> work;
     std::vector<int> cnt;
@@ -93,13 +91,12 @@ TeamLayout plan_teams(const std::vector<int> &n_tiles, const std::vector<int> &n
     return T;
 }
 
-std::vector<TeamLayout> plan_launches(const std::vector<int> &n_tiles,
-                                      const std::vector<int> &n_nodes, int cap, int resident)
+std::vector<TeamLayout> plan_launches(const std::vector<int> &team,
+                                      const std::vector<int> &n_nodes, int resident)
 {
-    const int count = (int)n_tiles.size();
+    const int count = (int)team.size();
     std::map<int, std::vector<int>, std::greater<int>> classes;
-    for (int k = 0; k < count; k++)
-        classes[team_size(n_tiles[k], std::min(cap, resident))].push_back(k);
+    for (int k = 0; k < count; k++) classes[team[k]].push_back(k);
     // first-fit of classes (largest team first) into launches
     std::vector<std::vector<int>> groups;
     std::vector<int> room;
@@ -123,10 +120,10 @@ std::vector<TeamLayout> plan_launches(const std::vector<int> &n_tiles,
         std::sort(grp.begin(), grp.end());
         std::vector<int> t, n;
         for (int k : grp) {
-            t.push_back(n_tiles[k]);
+            t.push_back(team[k]);
             n.push_back(n_nodes[k]);
         }
-        TeamLayout L = plan_teams(t, n, cap, resident);
+        TeamLayout L = plan_teams(t, n, resident);
         for (int &j : L.class_jobs) j = grp[j];
         std::vector<int> tm(count, 0);
         for (size_t i = 0; i < grp.size(); i++) tm[grp[i]] = L.team_of_net[i];
