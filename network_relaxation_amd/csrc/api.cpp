@@ -206,13 +206,13 @@ void nr_network_free(nr_network *net) { delete net; }
 // ---- batches --------------------------------------------------------------
 
 // Workgroup shape: `block` threads (256 or 512) each owning up to npt (1 or 2)
-// nodes of a tile; default 256 threads x 2 = 512-node tiles, two workgroups
-// per CU (LDS budget 80 KB each).
+// nodes of a tile; default 256 threads, one node each (256-node tiles), two
+// workgroups per CU (LDS budget 80 KB each).
 static int choose_shapes(nr_batch *b, const nr_batch_opts *opts, int64_t *lds_budget)
 {
     int block = (opts && opts->block > 0) ? opts->block : 256;
     if (block != 256 && block != 512) return fail(NR_E_ARG, "block must be 256 or 512");
-    int tile = (opts && opts->tile_nodes > 0) ? opts->tile_nodes : 2 * block;
+    int tile = (opts && opts->tile_nodes > 0) ? opts->tile_nodes : block;
     if (tile > 2 * block) return fail(NR_E_ARG, "tile_nodes above 2 x block");
     b->block = block;
     b->npt = tile > block ? 2 : 1;
