@@ -85,6 +85,9 @@ struct nr_batch {
     std::vector<nr::NetDev> hnet;
     std::vector<double> ks, kb;
     DevBuf<double2> x[2], v[2], f[2], x_init;
+    DevBuf<float2> u32[2], v32[2], f32[2];  // mixed precision FP32 stage
+    double switch_ratio = 1e-6;  // FP32 stage stops at this fraction of the initial |F|max
+    int stall_steps = 5000;      // ... or after this many steps without a new minimum
     DevBuf<unsigned char> mask;
     DevBuf<uint4> blob;
     DevBuf<nr::TileDev> tiles;
@@ -214,6 +217,8 @@ static int choose_shapes(nr_batch *b, const nr_batch_opts *opts, int64_t *lds_bu
     if (block != 256 && block != 512) return fail(NR_E_ARG, "block must be 256 or 512");
     int tile = (opts && opts->tile_nodes > 0) ? opts->tile_nodes : block;
     if (tile > 2 * block) return fail(NR_E_ARG, "tile_nodes above 2 x block");
+    if (b->precision == NR_PREC_MIXED && tile > block)
+        return fail(NR_E_ARG, "mixed precision needs tile_nodes <= block");
     b->block = block;
     b->npt = tile > block ? 2 : 1;
     b->plan.tile_nodes = tile;
@@ -228,7 +233,8 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     auto b = std::make_unique<nr_batch>();
     b->device = opts ? opts->device : 0;
     b->precision = opts ? opts->precision : NR_PREC_FP64;
-    if (b->precision != NR_PREC_FP64) return fail(NR_E_ARG, "precision not available");
+    if (b->precision != NR_PREC_FP64 && b->precision != NR_PREC_MIXED)
+        return fail(NR_E_ARG, "unknown precision");
     HIP_TRY(hipSetDevice(b->device));
     int64_t lds_budget = 0;
     int rc = choose_shapes(b.get(), opts, &lds_budget);
@@ -299,6 +305,14 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     }
     std::vector<double2> zeros(N, make_double2(0.0, 0.0));
     HIP_TRY(b->x_init.upload(x0));
+    if (b->precision == NR_PREC_MIXED) {
+        std::vector<float2> z32(N, make_float2(0.0f, 0.0f));
+        for (int s = 0; s < 2; s++) {
+            HIP_TRY(b->u32[s].upload(z32));
+            HIP_TRY(b->v32[s].upload(z32));
+            HIP_TRY(b->f32[s].upload(z32));
+        }
+    }
     for (int s = 0; s < 2; s++) {
         HIP_TRY(b->x[s].upload(x0));
         HIP_TRY(b->v[s].upload(zeros));
@@ -362,6 +376,14 @@ static int run_kernel(nr_batch *b, int mode)
         a.v[s] = b->v[s].p;
         a.f[s] = b->f[s].p;
     }
+    for (int s = 0; s < 2; s++) {
+        a.u32[s] = b->u32[s].p;
+        a.v32[s] = b->v32[s].p;
+        a.f32[s] = b->f32[s].p;
+    }
+    a.mixed = b->precision == NR_PREC_MIXED ? 1 : 0;
+    a.switch_ratio = b->switch_ratio;
+    a.stall_steps = b->stall_steps;
     a.mask = b->mask.p;
     a.blob = b->blob.p;
     a.tiles = b->tiles.p;

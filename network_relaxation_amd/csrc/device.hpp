@@ -42,6 +42,11 @@ struct BlockRole {
 
 struct KArgs {
     double2 *x[2], *v[2], *f[2];
+    // mixed precision: FP32 displacement from the FP64 anchor, velocity, force
+    float2 *u32[2], *v32[2], *f32[2];
+    int mixed;               // relax with an FP32 stage and an FP64 polish
+    double switch_ratio;     // FP32 stage ends at |F|max < switch_ratio * initial |F|max
+    int stall_steps;         // ... or after this many steps without a new minimum
     const unsigned char *mask;
     const uint4 *blob;
     const TileDev *tiles;

@@ -26,7 +26,9 @@ namespace nr {
 
 namespace {
 
-enum PassKind { kPassInit = 0, kPassStep = 1, kPassEnergy = 2 };
+// kPassFuse starts the FP64 polish of a mixed-precision relax: x = FP64
+// anchor + FP32 displacement, then forces as in kPassInit.
+enum PassKind { kPassInit = 0, kPassStep = 1, kPassEnergy = 2, kPassFuse = 3 };
 enum MixKind { kMixNone = 0, kMixFire = 1, kMixZero = 2 };
 
 struct Red4 {
@@ -38,6 +40,7 @@ struct StepScal {
     double alpha_old, cmix;  // deferred FIRE mix of the previous step
     int mix;
     double pend_x, pend_y;   // affine step (init pass)
+    int s;                   // parity of the FP32 state (mixed precision)
 };
 
 struct NetCtx {
@@ -84,6 +87,30 @@ __device__ inline void st_state(bool sh, double2 *p, double2 v)
         __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
         __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        *p = v;
+    }
+}
+
+__device__ inline float2 ld_state32(bool sh, const float2 *p)
+{
+    if (sh) {
+        const unsigned long long a = __hip_atomic_load(
+            reinterpret_cast<unsigned long long *>(const_cast<float2 *>(p)), __ATOMIC_RELAXED,
+            __HIP_MEMORY_SCOPE_AGENT);
+        return make_float2(__uint_as_float((unsigned)(a & 0xFFFFFFFFull)),
+                           __uint_as_float((unsigned)(a >> 32)));
+    }
+    return *p;
+}
+
+__device__ inline void st_state32(bool sh, float2 *p, float2 v)
+{
+    if (sh) {
+        const unsigned long long a = (unsigned long long)__float_as_uint(v.x) |
+                                     ((unsigned long long)__float_as_uint(v.y) << 32);
+        __hip_atomic_store(reinterpret_cast<unsigned long long *>(p), a, __ATOMIC_RELAXED,
                            __HIP_MEMORY_SCOPE_AGENT);
     } else {
         *p = v;
@@ -425,6 +452,270 @@ __device__ inline double2 gather_force(const Blob &bb, const double2 *fv, int k)
     return make_double2(fx, fy);
 }
 
+// FIRE scalar state and its per-step update (run_fire_md :616-622 and
+// :679-712): the mix of the step just taken is deferred to the next pass's
+// position update (S.alpha_old, S.cmix, S.mix).
+struct Fire {
+    double a0, fa, dtmax, finc, fdec, mass;
+    int nmin;
+    double alpha, dt, dtm, dtsq;
+    int since;
+
+    __device__ void start()
+    {
+        alpha = a0;
+        dt = dtmax;
+        dtm = 0.5 * dt / mass;
+        dtsq = 0.5 * dt * dt / mass;
+        since = 0;
+    }
+    __device__ void load(StepScal &S) const
+    {
+        S.dt = dt;
+        S.dtsq = dtsq;
+        S.dtm = dtm;
+    }
+    __device__ void update(const Red4 &r, StepScal &S)
+    {
+        const double P = r.a, vmag = sqrt(r.b), fmag = sqrt(r.c);
+        S.alpha_old = alpha;
+        S.cmix = alpha * vmag / fmag;
+        if (P > 0) {  // :694-703
+            since++;
+            if (since > nmin) {
+                dt = fmin(dt * finc, dtmax);
+                dtm = 0.5 * dt / mass;
+                dtsq = 0.5 * dt * dt / mass;
+                alpha *= fa;
+            }
+            S.mix = kMixFire;
+        } else {  // :704-712
+            since = 0;
+            dt *= fdec;
+            dtm = 0.5 * dt / mass;
+            dtsq = 0.5 * dt * dt / mass;
+            alpha = a0;
+            S.mix = kMixZero;
+        }
+    }
+};
+
+// x = FP64 anchor + FP32 displacement (start of the FP64 polish).
+__device__ inline double2 fuse_position(bool sh, const KArgs &A, const StepScal &S,
+                                        const double2 *X, int gid)
+{
+    const double2 a = ld_state(sh, X + gid);
+    const float2 u = ld_state32(sh, A.u32[S.s] + gid);
+    return make_double2(a.x + (double)u.x, a.y + (double)u.y);
+}
+
+// ---- FP32 stage of a mixed-precision relax ------------------------------
+// State: FP64 anchor a (positions after the strain step, fixed during the
+// stage) and FP32 displacement u, velocity v and force F.  Every geometric
+// term is split into an FP64 anchor part and FP32 displacement corrections,
+// so that FP32 rounding is relative to the displacement, not to the bond:
+//   bond:   d^2 - L^2 = (|a|^2 - L^2) + 2 a.du + du.du,
+//           f = ks (d^2 - L^2) / (d (d + L)) r      [= ks (1 - L/d) r]
+//   triple: cross = a_a x a_b + a_a x du_b + du_a x a_b + du_a x du_b,
+//           dot likewise, theta - pi = atan2(-cross, -dot).
+enum Pass32Kind { kPass32Init = 0, kPass32Step = 1 };
+
+struct Lds32 {
+    unsigned char *blob;
+    double2 *anc;  // FP64 anchors (own + halo)
+    float2 *u;     // FP32 displacements (own + halo)
+    float2 *fv;    // FP32 force vectors
+};
+
+template <int KIND>
+__device__ inline float2 advance32(const float2 u, const float2 v, const float2 f,
+                                   const StepScal &S, float2 &vh)
+{
+    if (KIND == kPass32Init) {
+        vh = make_float2(0.0f, 0.0f);
+        return make_float2(0.0f, 0.0f);
+    }
+    float2 vm;
+    if (S.mix == kMixFire) {
+        const float na = (float)-S.alpha_old, cm = (float)S.cmix;
+        vm.x = fmaf(cm, f.x, fmaf(na, v.x, v.x));
+        vm.y = fmaf(cm, f.y, fmaf(na, v.y, v.y));
+    } else if (S.mix == kMixZero) {
+        vm = make_float2(0.0f, 0.0f);
+    } else {
+        vm = v;
+    }
+    const float dt = (float)S.dt, dtsq = (float)S.dtsq, dtm = (float)S.dtm;
+    float2 un;
+    un.x = fmaf(dtsq, f.x, fmaf(dt, vm.x, u.x));
+    un.y = fmaf(dtsq, f.y, fmaf(dt, vm.y, u.y));
+    vh.x = fmaf(dtm, f.x, vm.x);
+    vh.y = fmaf(dtm, f.y, vm.y);
+    return un;
+}
+
+// One FP32-stage pass over this member's tiles.  Reads the anchor from x[pa]
+// (the init pass first writes it there from x[p] and the strain step) and the
+// FP32 state from parity S.s; writes the FP32 state to parity S.s ^ 1.
+template <int B, int KIND>
+__device__ Red4 pass32(const KArgs &A, const Team &T, const NetCtx &C, const StepScal &S,
+                       int p, int pa, const Lds &L0, bool resident)
+{
+    Red4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int tid = threadIdx.x;
+    const int s = S.s, s1 = S.s ^ 1;
+    const bool sh = T.g > 1;
+    const float2 zf = make_float2(0.0f, 0.0f);
+    const float ks = (float)C.ks;
+    for (int ti = T.member; ti < C.n_tiles; ti += T.g) {
+        const TileDev td = A.tiles[C.tile_base + ti];
+        const int local = td.n_own + td.n_halo;
+        Lds32 L;
+        L.blob = L0.blob;
+        L.anc = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
+        L.u = reinterpret_cast<float2 *>(L.anc + local);
+        L.fv = L.u + local;
+        if (!resident) {
+            stage_blob<B>(A, td, L.blob);
+            __syncthreads();
+        }
+        const Blob bb = blob_view(L.blob, td);
+        float2 vh = zf;
+        unsigned char mk = 0;
+        const int k = tid;
+        if (k < td.n_own) {
+            const int gid = td.node0 + k;
+            double2 a;
+            float2 un;
+            if (KIND == kPass32Init) {
+                double2 dummy;
+                a = advance<kPassInit>(ld_state(sh, A.x[p] + gid), make_double2(0.0, 0.0),
+                                       make_double2(0.0, 0.0), S, C, dummy);
+                st_state(sh, A.x[pa] + gid, a);
+                un = zf;
+            } else {
+                a = ld_state(sh, A.x[pa] + gid);
+                un = advance32<kPass32Step>(ld_state32(sh, A.u32[s] + gid),
+                                            ld_state32(sh, A.v32[s] + gid),
+                                            ld_state32(sh, A.f32[s] + gid), S, vh);
+            }
+            st_state32(sh, A.u32[s1] + gid, un);
+            mk = A.mask[gid];
+            L.anc[k] = a;
+            L.u[k] = un;
+        }
+        for (int kk = td.n_own + tid; kk < local; kk += B) {
+            const int gid = bb.halo[kk - td.n_own];
+            double2 a;
+            float2 un, dummy;
+            if (KIND == kPass32Init) {
+                double2 d2;
+                a = advance<kPassInit>(ld_state(sh, A.x[p] + gid), make_double2(0.0, 0.0),
+                                       make_double2(0.0, 0.0), S, C, d2);
+                un = zf;
+            } else {
+                a = ld_state(sh, A.x[pa] + gid);
+                un = advance32<kPass32Step>(ld_state32(sh, A.u32[s] + gid),
+                                            ld_state32(sh, A.v32[s] + gid),
+                                            ld_state32(sh, A.f32[s] + gid), S, dummy);
+            }
+            L.anc[kk] = a;
+            L.u[kk] = un;
+        }
+        __syncthreads();
+        for (int e = tid; e < td.nb; e += B) {
+            const uint32_t ij = bb.bij[e];
+            const uint32_t w = bb.bw[e];
+            const int i = (int)(ij & 0xFFFF), j = (int)(ij >> 16);
+            double ox, oy;
+            image(C, nib(w), nib(w >> 4), ox, oy);
+            const double2 ai = L.anc[i], aj = L.anc[j];
+            const double axd = aj.x + ox - ai.x, ayd = aj.y + oy - ai.y;
+            const double L64 = bb.bl[e];
+            const float s0 = (float)(fma(axd, axd, ayd * ayd) - L64 * L64);
+            const float2 ui = L.u[i], uj = L.u[j];
+            const float dx = uj.x - ui.x, dy = uj.y - ui.y;
+            const float ax = (float)axd, ay = (float)ayd;
+            const float rx = ax + dx, ry = ay + dy;
+            const float d2ml2 = s0 + fmaf(2.0f * ax + dx, dx, (2.0f * ay + dy) * dy);
+            const float d = sqrtf(fmaf(rx, rx, ry * ry));
+            const float coef = ks * d2ml2 / (d * (d + (float)L64));
+            L.fv[e] = make_float2(coef * rx, coef * ry);
+        }
+        for (int t = tid; t < td.nt; t += B) {
+            const uint64_t code = bb.tc[t];
+            const int h = (int)(code & 0xFFFF), a = (int)((code >> 16) & 0xFFFF),
+                      c = (int)((code >> 32) & 0xFFFF);
+            const uint32_t w = (uint32_t)(code >> 48);
+            double oxa, oya, oxb, oyb;
+            image(C, nib(w), nib(w >> 4), oxa, oya);
+            image(C, nib(w >> 8), nib(w >> 12), oxb, oyb);
+            const double2 ah = L.anc[h], aa = L.anc[a], ac = L.anc[c];
+            const double pax = aa.x + oxa - ah.x, pay = aa.y + oya - ah.y;
+            const double pbx = ac.x + oxb - ah.x, pby = ac.y + oyb - ah.y;
+            const float cr0 = (float)fma(pax, pby, -(pay * pbx));
+            const float dt0 = (float)fma(pax, pbx, pay * pby);
+            const float2 uh = L.u[h], ua = L.u[a], uc = L.u[c];
+            const float dax = ua.x - uh.x, day = ua.y - uh.y;
+            const float dbx = uc.x - uh.x, dby = uc.y - uh.y;
+            const float fax = (float)pax, fay = (float)pay, fbx = (float)pbx, fby = (float)pby;
+            const float cr = cr0 + ((fax * dby - fay * dbx) + (dax * fby - day * fbx) +
+                                    (dax * dby - day * dbx));
+            const float dt = dt0 + ((fax * dbx + fay * dby) + (dax * fbx + day * fby) +
+                                    (dax * dbx + day * dby));
+            const float dth = atan2f(-cr, -dt);
+            const float rax = fax + dax, ray = fay + day, rbx = fbx + dbx, rby = fby + dby;
+            const float g = (float)bb.tk[t] * dth;
+            const float ia = g / fmaf(rax, rax, ray * ray);
+            const float ib = g / fmaf(rbx, rbx, rby * rby);
+            const float2 f2 = make_float2(ia * -ray, ia * rax);
+            const float2 f3 = make_float2(ib * rby, -ib * rbx);
+            L.fv[td.nb + t] = f2;
+            L.fv[td.nb + td.nt + t] = f3;
+            L.fv[td.nb + 2 * td.nt + t] = make_float2(-f2.x - f3.x, -f2.y - f3.y);
+        }
+        __syncthreads();
+        if (k < td.n_own) {
+            const int gid = td.node0 + k;
+            double fx = 0.0, fy = 0.0;
+            const int lo = bb.ioff[k], hi = bb.ioff[k + 1];
+            for (int q = lo; q < hi; q++) {
+                const uint32_t e = bb.inc[q];
+                const float2 v = L.fv[e & kMaxForceVectors];
+                if (e & kIncNegate) {
+                    fx -= (double)v.x;
+                    fy -= (double)v.y;
+                } else {
+                    fx += (double)v.x;
+                    fy += (double)v.y;
+                }
+            }
+            if (!(mk & 1)) fx = 0.0;
+            if (!(mk & 2)) fy = 0.0;
+            const float2 fn = make_float2((float)fx, (float)fy);
+            float2 vn;
+            if (KIND == kPass32Init) {
+                vn = zf;
+            } else {
+                const float dtm = (float)S.dtm;
+                vn = make_float2(fmaf(dtm, fn.x, vh.x), fmaf(dtm, fn.y, vh.y));
+            }
+            st_state32(sh, A.f32[s1] + gid, fn);
+            st_state32(sh, A.v32[s1] + gid, vn);
+            const double vx = vn.x, vy = vn.y, gx = fn.x, gy = fn.y;
+            acc.a = fma(vx, gx, acc.a);
+            acc.a = fma(vy, gy, acc.a);
+            acc.b = fma(vx, vx, acc.b);
+            acc.b = fma(vy, vy, acc.b);
+            const double f2 = fma(gx, gx, gy * gy);
+            acc.c += f2;
+            acc.m = fmax(acc.m, f2);
+        }
+        __syncthreads();
+    }
+    return acc;
+}
+
 // One pass over this member's tiles.  Init/step passes write x, v, F into
 // buffer q = 1 - p and return (P, |v|^2, |F|^2, max |F_node|^2) partials;
 // the energy pass reads buffer p and returns (E_stretch, E_bend, 0, 0).
@@ -468,6 +759,9 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 } else if (KIND == kPassInit) {
                     xn = advance<kPassInit>(ld_state(sh, X + gid), zero, zero, S, C, vh[j]);
                     st_state(sh, A.x[q] + gid, xn);
+                } else if (KIND == kPassFuse) {
+                    xn = fuse_position(sh, A, S, X, gid);
+                    st_state(sh, A.x[q] + gid, xn);
                 } else {
                     xn = advance<kPassStep>(ld_state(sh, X + gid), ld_state(sh, V + gid),
                                             ld_state(sh, F + gid), S, C, vh[j]);
@@ -484,6 +778,8 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 xn = ld_state(sh, X + gid);
             else if (KIND == kPassInit)
                 xn = advance<kPassInit>(ld_state(sh, X + gid), zero, zero, S, C, dummy);
+            else if (KIND == kPassFuse)
+                xn = fuse_position(sh, A, S, X, gid);
             else
                 xn = advance<kPassStep>(ld_state(sh, X + gid), ld_state(sh, V + gid),
                                         ld_state(sh, F + gid), S, C, dummy);
@@ -548,7 +844,7 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                     double2 vn;
                     vn.x = fma(S.dtm, fn.x, vh[j].x);  // :677 second half kick
                     vn.y = fma(S.dtm, fn.y, vh[j].y);
-                    if (KIND == kPassInit) vn = zero;
+                    if (KIND == kPassInit || KIND == kPassFuse) vn = zero;
                     st_state(sh, A.f[q] + gid, fn);
                     st_state(sh, A.v[q] + gid, vn);
                     acc.a = fma(vn.x, fn.x, acc.a);
@@ -608,55 +904,82 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     S.mix = kMixNone;
     S.pend_x = pend_x;
     S.pend_y = pend_y;
+    S.s = 0;
 
     int status = NR_NOT_CONVERGED;
     Red4 r = {0.0, 0.0, 0.0, 0.0};
-    Red4 part = pass<B, NPT, kPassInit>(A, T, C, S, p, L, resident);
-    if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+    Red4 part;
+    const double sqrt_dof = sqrt((double)(2 * C.n_nodes));
+    Fire fire;
+    fire.a0 = a0;
+    fire.fa = fa;
+    fire.dtmax = dtmax;
+    fire.finc = finc;
+    fire.fdec = fdec;
+    fire.mass = mass;
+    fire.nmin = nmin;
+    int step = 0, steps32 = 0;
+
+    if (A.mixed && max_steps > 0) {
+        // FP32 stage: anchor = positions after the strain step, in x[p ^ 1]
+        const int pa = p ^ 1;
+        S.s = 0;
+        part = pass32<B, kPass32Init>(A, T, C, S, p, pa, L, resident);
+        if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+        S.s = 1;
+        const double fswitch = fmax(fcut, A.switch_ratio * sqrt(r.m));
+        double best = r.m;
+        int since_best = 0;
+        fire.start();
+        S.mix = kMixNone;
+        while (status != NR_ABORTED && step < max_steps) {
+            step++;
+            fire.load(S);
+            part = pass32<B, kPass32Step>(A, T, C, S, p, pa, L, resident);
+            if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
+                status = NR_ABORTED;
+                break;
+            }
+            S.s ^= 1;
+            fire.update(r, S);
+            if (!isfinite(r.c) || sqrt(r.m) < fswitch) break;
+            if (r.m < best) {
+                best = r.m;
+                since_best = 0;
+            } else if (++since_best > A.stall_steps) {
+                break;
+            }
+        }
+        steps32 = step;
+        // FP64 polish from anchor + displacement
+        p = pa;
+        S.pend_x = 0.0;
+        S.pend_y = 0.0;
+        if (status != NR_ABORTED) part = pass<B, NPT, kPassFuse>(A, T, C, S, p, L, resident);
+    } else {
+        part = pass<B, NPT, kPassInit>(A, T, C, S, p, L, resident);
+    }
+    if (status != NR_ABORTED && !reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
     p ^= 1;
     double ff = r.c, fmax2 = r.m;
     double fmag = sqrt(ff);
-    const double sqrt_dof = sqrt((double)(2 * C.n_nodes));
 
     // run_fire_md :616-622
-    double alpha = a0, dt = dtmax;
-    double dtm = 0.5 * dt / mass, dtsq = 0.5 * dt * dt / mass;
-    int since = 0, step = 0;
+    fire.start();
+    S.mix = kMixNone;
     while (status != NR_ABORTED && step < max_steps) {
         step++;
-        S.dt = dt;
-        S.dtsq = dtsq;
-        S.dtm = dtm;
+        fire.load(S);
         part = pass<B, NPT, kPassStep>(A, T, C, S, p, L, resident);
         if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
             status = NR_ABORTED;
             break;
         }
         p ^= 1;
-        const double P = r.a, vv = r.b;
         ff = r.c;
         fmax2 = r.m;
-        const double vmag = sqrt(vv);
         fmag = sqrt(ff);
-        S.alpha_old = alpha;
-        S.cmix = alpha * vmag / fmag;
-        if (P > 0) {  // :694-703
-            since++;
-            if (since > nmin) {
-                dt = fmin(dt * finc, dtmax);
-                dtm = 0.5 * dt / mass;
-                dtsq = 0.5 * dt * dt / mass;
-                alpha *= fa;
-            }
-            S.mix = kMixFire;
-        } else {  // :704-712
-            since = 0;
-            dt *= fdec;
-            dtm = 0.5 * dt / mass;
-            dtsq = 0.5 * dt * dt / mass;
-            alpha = a0;
-            S.mix = kMixZero;
-        }
+        fire.update(r, S);
         const bool conv = crit == NR_CRIT_MAX ? sqrt(fmax2) < fcut : fmag / sqrt_dof < fcut;
         if (conv) break;
         if (!isfinite(ff)) break;
@@ -683,7 +1006,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
         o.eb = C.kb > 0 ? e.b : 0.0;
         o.steps = step;
         o.status = status;
-        o.fp64_steps = step;
+        o.fp64_steps = step - steps32;
         o.pad = 0;
         o.t_start = t_start;
         o.t_done = wall_clock64();

@@ -1,0 +1,63 @@
+"""GPU: the nrelax CLI (lsrelax drop-in) against the oracle's protocol run on
+the same batch file and network (reference mode: x-periodic, clamped rows)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+import network_relaxation_amd as N
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _write_dat(net, path):
+    a = net.arrays()
+    i = net.info()
+    with open(path, "w") as f:
+        f.write(f"{i.n_nodes} {i.n_bonds} {i.width!r}\n")
+        for x, y in a["pos"].reshape(-1, 2):
+            f.write(f"{float(x)!r} {float(y)!r}\n")
+        for (p, q), (w, _) in zip(a["bonds"], a["bond_wraps"]):
+            f.write(f"{p} {q} {w}\n")
+
+
+def test_cli_protocol_matches_oracle(tmp_path):
+    print("case cli protocol", flush=True)
+    gpu_dir, cpu_dir = tmp_path / "gpu", tmp_path / "cpu"
+    gpu_dir.mkdir()
+    cpu_dir.mkdir()
+    net = N.Network.lattice(16, 0.8, 9, False)
+    batch = ("netA t\n1 0.01\n0.1 0.99 0.1 1.1 0.5\n5 1 1e-10 200000\n"
+             "0.01 0.02 0.01\n0.01 0.03 0.01\n")
+    for d in (gpu_dir, cpu_dir):
+        _write_dat(net, str(d / "netA.dat"))
+        (d / "batch.txt").write_text(batch.replace("netA", str(d / "netA")))
+    res = subprocess.run([N.CLI_PATH, "-b", str(gpu_dir / "batch.txt")], capture_output=True,
+                         text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    rc, base, tag, b = O.parse_batch(str(cpu_dir / "batch.txt"))
+    assert rc == 0
+    stem = str(cpu_dir / "netA_t")
+    assert O.run_protocol(str(cpu_dir / "netA.dat"), stem + "_log.txt", stem + "_edata.txt",
+                          stem, b) == 0
+    g_log = (gpu_dir / "netA_t_log.txt").read_text().splitlines()
+    c_log = (cpu_dir / "netA_t_log.txt").read_text().splitlines()
+    assert len(g_log) == len(c_log) == 6
+    for gl, cl in zip(g_log[:-1], c_log[:-1]):
+        assert gl.split("Steps")[0] == cl.split("Steps")[0]   # same convergence verdict
+    g_ed = (gpu_dir / "netA_t_edata.txt").read_text().splitlines()
+    c_ed = (cpu_dir / "netA_t_edata.txt").read_text().splitlines()
+    assert len(g_ed) == len(c_ed) == 5
+    for gl, cl in zip(g_ed, c_ed):
+        g, c = [float(x) for x in gl.split("\t")], [float(x) for x in cl.split("\t")]
+        assert g[:2] == c[:2] and gl.split("\t")[:2] == cl.split("\t")[:2]
+        for a, e in zip(g[2:], c[2:]):
+            assert abs(a - e) <= 1e-9 * abs(e) + 1e-18
+    for k in (1, 2, 3):
+        gp = np.loadtxt(gpu_dir / f"netA_t_{k}.pos")
+        cp = np.loadtxt(cpu_dir / f"netA_t_{k}.pos")
+        assert gp.shape == cp.shape
+        np.testing.assert_allclose(gp, cp, rtol=0, atol=1e-8)
