@@ -94,6 +94,9 @@ typedef struct {
     int device;        /* HIP device ordinal */
     int tiles_per_member; /* a network of T tiles gets a team of about
                              T / tiles_per_member workgroups, 0 = tuned */
+    double mixed_switch;  /* NR_PREC_MIXED: the FP32 stage hands over to FP64
+                             when max |F| < mixed_switch x its initial value,
+                             0 = default (1e-4) */
 } nr_batch_opts;
 
 /* ---- errors ------------------------------------------------------------ */

@@ -50,7 +50,8 @@ class Report(C.Structure):
 
 class BatchOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("block", C.c_int), ("tile_nodes", C.c_int),
-                ("team", C.c_int), ("device", C.c_int), ("tiles_per_member", C.c_int)]
+                ("team", C.c_int), ("device", C.c_int), ("tiles_per_member", C.c_int),
+                ("mixed_switch", C.c_double)]
 
 
 class NRError(RuntimeError):
@@ -192,13 +193,14 @@ class Batch:
     """Device-resident networks (nr_batch) relaxed by one persistent launch."""
 
     def __init__(self, networks, ks, kb, precision=PREC_FP64, tile_nodes=0, team=0, device=0,
-                 block=0, tiles_per_member=0):
+                 block=0, tiles_per_member=0, mixed_switch=0.0):
         self.networks = list(networks)  # keep the host networks alive
         n = len(self.networks)
         ks = np.broadcast_to(np.asarray(ks, np.float64), (n,)).copy()
         kb = np.broadcast_to(np.asarray(kb, np.float64), (n,)).copy()
         handles = (C.c_void_p * n)(*[x.h.value for x in self.networks])
-        opts = BatchOpts(precision, block, tile_nodes, team, device, tiles_per_member)
+        opts = BatchOpts(precision, block, tile_nodes, team, device, tiles_per_member,
+                         mixed_switch)
         h = C.c_void_p()
         _check(lib().nr_batch_create(n, handles, _dp(ks), _dp(kb), C.byref(opts), C.byref(h)))
         self.h = h
@@ -259,6 +261,6 @@ class Batch:
 
 def relaxation_batch(batch_file, strain_mode=True, precision=PREC_FP64, tile_nodes=0):
     """nr_relaxation_batch: the reference program's whole protocol."""
-    opts = BatchOpts(precision, 0, tile_nodes, 0, 0, 0)
+    opts = BatchOpts(precision, 0, tile_nodes, 0, 0, 0, 0.0)
     return lib().nr_relaxation_batch(os.fsencode(batch_file), 1 if strain_mode else 0,
                                      C.byref(opts))

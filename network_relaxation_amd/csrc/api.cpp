@@ -86,8 +86,8 @@ struct nr_batch {
     std::vector<double> ks, kb;
     DevBuf<double2> x[2], v[2], f[2], x_init;
     DevBuf<float2> u32[2], v32[2], f32[2];  // mixed precision FP32 stage
-    double switch_ratio = 1e-6;  // FP32 stage stops at this fraction of the initial |F|max
-    int stall_steps = 5000;      // ... or after this many steps without a new minimum
+    double switch_ratio = 1e-4;  // FP32 stage stops at this fraction of the initial |F|max
+    int stall_steps = 2000;      // ... or after this many steps without a new minimum
     DevBuf<unsigned char> mask;
     DevBuf<uint4> blob;
     DevBuf<nr::TileDev> tiles;
@@ -233,6 +233,7 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     auto b = std::make_unique<nr_batch>();
     b->device = opts ? opts->device : 0;
     b->precision = opts ? opts->precision : NR_PREC_FP64;
+    if (opts && opts->mixed_switch > 0) b->switch_ratio = opts->mixed_switch;
     if (b->precision != NR_PREC_FP64 && b->precision != NR_PREC_MIXED)
         return fail(NR_E_ARG, "unknown precision");
     HIP_TRY(hipSetDevice(b->device));

@@ -319,6 +319,9 @@ __device__ inline Blob blob_view(const unsigned char *base, const TileDev &td)
     return b;
 }
 
+// 16-byte blob loads a thread issues before the first of them is waited on.
+constexpr int kBlobRound = 8;
+
 // Copies a tile blob into LDS: 16-byte loads, four in flight per thread.
 template <int B>
 __device__ inline void stage_blob(const KArgs &A, const TileDev &td, unsigned char *dst_bytes)
@@ -737,40 +740,86 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
         Lds L = L0;
         L.pos = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
         L.fv = L.pos + td.n_own + td.n_halo;
+        // The blob's first kBlobRound x 16 B per thread and the own nodes'
+        // state are loaded together; the LDS writes then wait only for the
+        // loads they use.
+        const uint4 *__restrict__ bsrc = A.blob + td.blob16;
+        uint4 *bdst = reinterpret_cast<uint4 *>(L.blob);
+        const int n16 = resident ? 0 : td.blob_bytes >> 4;
+        uint4 r0, r1, r2, r3, r4, r5, r6, r7;
         if (!resident) {
-            stage_blob<B>(A, td, L.blob);
-            __syncthreads();
+            const int last = n16 - 1;
+            r0 = bsrc[min(tid, last)];
+            r1 = bsrc[min(tid + B, last)];
+            r2 = bsrc[min(tid + 2 * B, last)];
+            r3 = bsrc[min(tid + 3 * B, last)];
+            r4 = bsrc[min(tid + 4 * B, last)];
+            r5 = bsrc[min(tid + 5 * B, last)];
+            r6 = bsrc[min(tid + 6 * B, last)];
+            r7 = bsrc[min(tid + 7 * B, last)];
         }
-        const Blob bb = blob_view(L.blob, td);
-        // positions: own nodes keep their half-kicked velocity and clamp mask
-        // in registers
-        double2 vh[NPT];
+        double2 xo[NPT], vo[NPT], fo[NPT];
         unsigned char mk[NPT];
 #pragma unroll
         for (int j = 0; j < NPT; j++) {
             const int k = tid + j * B;
-            vh[j] = zero;
+            xo[j] = vo[j] = fo[j] = zero;
             mk[j] = 0;
+            if (k < td.n_own) {
+                const int gid = td.node0 + k;
+                if (KIND == kPassFuse) {
+                    xo[j] = fuse_position(sh, A, S, X, gid);
+                    const float2 v32 = ld_state32(sh, A.v32[S.s] + gid);
+                    vo[j] = make_double2((double)v32.x, (double)v32.y);
+                } else {
+                    xo[j] = ld_state(sh, X + gid);
+                }
+                if (KIND == kPassStep) {
+                    vo[j] = ld_state(sh, V + gid);
+                    fo[j] = ld_state(sh, F + gid);
+                }
+                if (KIND != kPassEnergy) mk[j] = A.mask[gid];
+            }
+        }
+        if (!resident) {
+            if (tid < n16) bdst[tid] = r0;
+            if (tid + B < n16) bdst[tid + B] = r1;
+            if (tid + 2 * B < n16) bdst[tid + 2 * B] = r2;
+            if (tid + 3 * B < n16) bdst[tid + 3 * B] = r3;
+            if (tid + 4 * B < n16) bdst[tid + 4 * B] = r4;
+            if (tid + 5 * B < n16) bdst[tid + 5 * B] = r5;
+            if (tid + 6 * B < n16) bdst[tid + 6 * B] = r6;
+            if (tid + 7 * B < n16) bdst[tid + 7 * B] = r7;
+            for (int i = kBlobRound * B + tid; i < n16; i += B) bdst[i] = bsrc[i];
+        }
+        // positions: own nodes keep their half-kicked velocity and clamp mask
+        // in registers
+        double2 vh[NPT];
+#pragma unroll
+        for (int j = 0; j < NPT; j++) {
+            const int k = tid + j * B;
+            vh[j] = zero;
             if (k < td.n_own) {
                 const int gid = td.node0 + k;
                 double2 xn;
                 if (KIND == kPassEnergy) {
-                    xn = ld_state(sh, X + gid);
+                    xn = xo[j];
                 } else if (KIND == kPassInit) {
-                    xn = advance<kPassInit>(ld_state(sh, X + gid), zero, zero, S, C, vh[j]);
+                    xn = advance<kPassInit>(xo[j], zero, zero, S, C, vh[j]);
                     st_state(sh, A.x[q] + gid, xn);
                 } else if (KIND == kPassFuse) {
-                    xn = fuse_position(sh, A, S, X, gid);
+                    xn = xo[j];
+                    vh[j] = vo[j];
                     st_state(sh, A.x[q] + gid, xn);
                 } else {
-                    xn = advance<kPassStep>(ld_state(sh, X + gid), ld_state(sh, V + gid),
-                                            ld_state(sh, F + gid), S, C, vh[j]);
+                    xn = advance<kPassStep>(xo[j], vo[j], fo[j], S, C, vh[j]);
                     st_state(sh, A.x[q] + gid, xn);
                 }
-                if (KIND != kPassEnergy) mk[j] = A.mask[gid];
                 L.pos[k] = xn;
             }
         }
+        if (!resident) __syncthreads();  // halo ids come from the blob
+        const Blob bb = blob_view(L.blob, td);
         for (int k = td.n_own + tid; k < td.n_own + td.n_halo; k += B) {
             const int gid = bb.halo[k - td.n_own];
             double2 dummy, xn;
@@ -844,7 +893,8 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                     double2 vn;
                     vn.x = fma(S.dtm, fn.x, vh[j].x);  // :677 second half kick
                     vn.y = fma(S.dtm, fn.y, vh[j].y);
-                    if (KIND == kPassInit || KIND == kPassFuse) vn = zero;
+                    if (KIND == kPassInit) vn = zero;
+                    if (KIND == kPassFuse) vn = vh[j];  // the FP32 stage's velocity
                     st_state(sh, A.f[q] + gid, fn);
                     st_state(sh, A.v[q] + gid, vn);
                     acc.a = fma(vn.x, fn.x, acc.a);
@@ -964,9 +1014,12 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     double ff = r.c, fmax2 = r.m;
     double fmag = sqrt(ff);
 
-    // run_fire_md :616-622
-    fire.start();
-    S.mix = kMixNone;
+    // run_fire_md :616-622 (a mixed-precision relax continues the FP32
+    // stage's FIRE state, its velocity having been carried over by the fuse)
+    if (!(A.mixed && steps32 > 0)) {
+        fire.start();
+        S.mix = kMixNone;
+    }
     while (status != NR_ABORTED && step < max_steps) {
         step++;
         fire.load(S);
