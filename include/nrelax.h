@@ -38,6 +38,8 @@ extern "C" {
 #define NR_NOT_CONVERGED 1   /* step limit reached (logged, not an error) */
 #define NR_NONFINITE 2       /* force norm became NaN/Inf */
 #define NR_ABORTED 3         /* device wait bound exceeded */
+#define NR_IN_PROGRESS 4     /* serving: step budget of a launch reached, the
+                                relax continues in the next launch */
 
 #define NR_CRIT_RMS 0        /* |F|_2 / sqrt(2N) < fcut (reference :716) */
 #define NR_CRIT_MAX 1        /* max over nodes of |F_node| < fcut */
@@ -150,6 +152,15 @@ int nr_batch_strain(nr_batch *b, int index, double xdisp, double ydisp);
  * when shared != 0.  reports[count] receives the outcomes. */
 int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared,
                    nr_report *reports);
+/* Strain ramp of do_relaxation_run (large_strain_relax.cu:917-1000, the
+ * compression then the shear loop) for every network in ONE launch: the
+ * networks take the n_inc affine steps incs[2i] (xdisp), incs[2i+1] (ydisp)
+ * in turn, each relaxed (run_fire_md) from the previous relaxed state, with
+ * no host round trip between increments.  reports receives count * n_inc
+ * outcomes, network-major; a network whose relax turns non-finite stops its
+ * ramp there (its later reports are left untouched). */
+int nr_batch_ramp(nr_batch *b, int n_inc, const double *incs,
+                  const nr_relax_params *params, int shared, nr_report *reports);
 /* Restores every network to its input geometry (zero strain, no pending
  * step) without host traffic. */
 int nr_batch_reset(nr_batch *b);
@@ -174,6 +185,78 @@ void nr_batch_destroy(nr_batch *b);
  * strain_mode != 0 scales displacements by the y range (the default). */
 int nr_relaxation_batch(const char *batch_file, int strain_mode,
                         const nr_batch_opts *opts);
+
+/* ---- serving ------------------------------------------------------------ */
+
+/* A request loop with continuous batching on one GPU; no counterpart in the
+ * reference, which relaxes one network at a time.  Jobs are laid out on host
+ * worker threads, admitted as device memory and slots free up (arrival order;
+ * smaller jobs fill in behind one that does not fit yet, until it has waited
+ * a second), and relaxed by repeated launches of the persistent kernel that
+ * give every resident job at most slice_steps FIRE steps.  Each job's state
+ * stays on the device across launches and across its strain increments.  A
+ * job's arithmetic does not depend on the other resident jobs, so its result
+ * is the same alone and in any mix.  One thread drives a server. */
+typedef struct nr_server nr_server;
+
+typedef struct {
+    int64_t hbm_bytes;    /* device arena for resident jobs, 0 = 8 GiB */
+    int max_jobs;         /* resident job slots, 0 = 4096 */
+    int slice_steps;      /* FIRE steps per job per launch, 0 = 2000 */
+    int planner_threads;  /* host threads building device layouts, 0 = 4 */
+} nr_server_opts;
+
+typedef struct {
+    int job;              /* id from nr_server_submit */
+    int status;           /* NR_CONVERGED when every increment converged, else
+                             the first other outcome (NR_ABORTED with `error`
+                             set when the job could not be laid out or placed) */
+    int error;            /* NR_OK or an NR_E_ code */
+    int n_inc;            /* increments with a report */
+    int steps;            /* FIRE steps over all increments */
+    int n_nodes;
+    double t_submit, t_admit, t_done;     /* nr_server_clock() seconds */
+    double e_stretch, e_bend, max_force;  /* after the last increment */
+} nr_job_result;
+
+typedef struct {
+    int64_t slices;       /* launches so far */
+    double busy_ms;       /* device time of those launches */
+    int64_t node_cap, nodes_free;
+    int resident_workgroups;
+    int active, waiting, done;
+} nr_server_info;
+
+/* shape: precision, block, tile_nodes, team, device, tiles_per_member,
+ * mixed_switch as for nr_batch_create (NULL = defaults). */
+int nr_server_create(const nr_batch_opts *shape, const nr_server_opts *opts,
+                     nr_server **out);
+/* Seconds since the server was created (the clock of nr_job_result). */
+double nr_server_clock(const nr_server *s);
+/* Queues a job: the network is copied (triples built when kb > 0); it takes
+ * the n_inc affine steps incs[2i] (xdisp), incs[2i+1] (ydisp) in turn, each
+ * relaxed from the previous relaxed state (the loop of do_relaxation_run,
+ * large_strain_relax.cu:917-1000).  keep_positions != 0 keeps the final
+ * positions for nr_server_result.  *job receives the id. */
+int nr_server_submit(nr_server *s, const nr_network *net, double ks, double kb,
+                     const nr_relax_params *params, int n_inc, const double *incs,
+                     int keep_positions, int *job);
+/* One scheduling round: waits for the launch in flight, retires the jobs it
+ * finished, starts the next launch over the resident jobs, then admits
+ * waiting jobs (their uploads overlap that launch; they join the next one).
+ * Returns once the launch is started. */
+int nr_server_step(nr_server *s, int *n_active, int *n_waiting);
+/* Steps until every submitted job is finished. */
+int nr_server_drain(nr_server *s);
+/* NR_OK and the id of the oldest finished job not yet taken, or 1 if none. */
+int nr_server_next_done(nr_server *s, int *job);
+/* Copies a finished job's result, its n_inc reports (reports may be NULL)
+ * and, when kept, its final positions pos[2N] (may be NULL), then forgets
+ * the job. */
+int nr_server_result(nr_server *s, int job, nr_job_result *res, nr_report *reports,
+                     double *pos);
+int nr_server_stats(const nr_server *s, nr_server_info *st);
+void nr_server_destroy(nr_server *s);
 
 #ifdef __cplusplus
 }

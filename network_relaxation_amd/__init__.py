@@ -17,7 +17,7 @@ CLI_PATH = os.path.join(_HERE, "nrelax")
 
 OK = 0
 E_OPEN, E_HEADER, E_POINTS, E_INDEX, E_EDGES, E_ALLOC, E_ARG, E_HIP, E_TOPO, E_BATCH, E_TIMEOUT = range(1, 12)
-CONVERGED, NOT_CONVERGED, NONFINITE, ABORTED = range(4)
+CONVERGED, NOT_CONVERGED, NONFINITE, ABORTED, IN_PROGRESS = range(5)
 CRIT_RMS, CRIT_MAX = 0, 1
 METHOD_FIRE, METHOD_CG = 0, 1
 PREC_FP64, PREC_MIXED = 0, 1
@@ -52,6 +52,24 @@ class BatchOpts(C.Structure):
     _fields_ = [("precision", C.c_int), ("block", C.c_int), ("tile_nodes", C.c_int),
                 ("team", C.c_int), ("device", C.c_int), ("tiles_per_member", C.c_int),
                 ("mixed_switch", C.c_double)]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [("hbm_bytes", C.c_int64), ("max_jobs", C.c_int), ("slice_steps", C.c_int),
+                ("planner_threads", C.c_int)]
+
+
+class JobResult(C.Structure):
+    _fields_ = [("job", C.c_int), ("status", C.c_int), ("error", C.c_int), ("n_inc", C.c_int),
+                ("steps", C.c_int), ("n_nodes", C.c_int), ("t_submit", C.c_double),
+                ("t_admit", C.c_double), ("t_done", C.c_double), ("e_stretch", C.c_double),
+                ("e_bend", C.c_double), ("max_force", C.c_double)]
+
+
+class ServerInfo(C.Structure):
+    _fields_ = [("slices", C.c_int64), ("busy_ms", C.c_double), ("node_cap", C.c_int64),
+                ("nodes_free", C.c_int64), ("resident_workgroups", C.c_int),
+                ("active", C.c_int), ("waiting", C.c_int), ("done", C.c_int)]
 
 
 class NRError(RuntimeError):
@@ -98,6 +116,18 @@ def lib():
         L.nr_batch_destroy.argtypes = [vp]
         L.nr_batch_destroy.restype = None
         L.nr_relaxation_batch.argtypes = [C.c_char_p, C.c_int, P(BatchOpts)]
+        L.nr_batch_ramp.argtypes = [vp, C.c_int, dp, P(RelaxParams), C.c_int, P(Report)]
+        L.nr_server_create.argtypes = [P(BatchOpts), P(ServerOpts), P(vp)]
+        L.nr_server_clock.argtypes = [vp]
+        L.nr_server_clock.restype = C.c_double
+        L.nr_server_submit.argtypes = [vp, vp, C.c_double, C.c_double, P(RelaxParams), C.c_int,
+                                       dp, C.c_int, ip]
+        L.nr_server_step.argtypes = [vp, ip, ip]
+        L.nr_server_drain.argtypes = [vp]
+        L.nr_server_next_done.argtypes = [vp, ip]
+        L.nr_server_result.argtypes = [vp, C.c_int, P(JobResult), P(Report), dp]
+        L.nr_server_stats.argtypes = [vp, P(ServerInfo)]
+        L.nr_server_destroy.argtypes = [vp]
         _lib = L
     return _lib
 
@@ -232,6 +262,19 @@ class Batch:
             _check(lib().nr_batch_relax(self.h, arr, 0, reps))
         return [r.as_dict() for r in reps]
 
+    def ramp(self, incs, params):
+        """nr_batch_ramp: incs is (n_inc, 2) of (xdisp, ydisp) for every network;
+        returns reports[count][n_inc]."""
+        incs = np.ascontiguousarray(incs, np.float64).reshape(-1, 2)
+        n = len(incs)
+        reps = (Report * (self.count * n))()
+        if isinstance(params, RelaxParams):
+            _check(lib().nr_batch_ramp(self.h, n, _dp(incs), C.byref(params), 1, reps))
+        else:
+            arr = (RelaxParams * self.count)(*params)
+            _check(lib().nr_batch_ramp(self.h, n, _dp(incs), arr, 0, reps))
+        return [[reps[k * n + i].as_dict() for i in range(n)] for k in range(self.count)]
+
     def reset(self):
         _check(lib().nr_batch_reset(self.h))
 
@@ -257,6 +300,71 @@ class Batch:
         n, e, t = C.c_int64(), C.c_int64(), C.c_int64()
         _check(lib().nr_batch_counts(self.h, index, C.byref(n), C.byref(e), C.byref(t)))
         return n.value, e.value, t.value
+
+
+class Server:
+    """nr_server: continuous batching of relax jobs on one GPU."""
+
+    def __init__(self, precision=PREC_FP64, tile_nodes=0, team=0, device=0, block=0,
+                 tiles_per_member=0, mixed_switch=0.0, hbm_bytes=0, max_jobs=0, slice_steps=0,
+                 planner_threads=0):
+        shape = BatchOpts(precision, block, tile_nodes, team, device, tiles_per_member,
+                          mixed_switch)
+        so = ServerOpts(hbm_bytes, max_jobs, slice_steps, planner_threads)
+        h = C.c_void_p()
+        _check(lib().nr_server_create(C.byref(shape), C.byref(so), C.byref(h)))
+        self.h = h
+        self.meta = {}
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.nr_server_destroy(self.h)
+            self.h = None
+
+    def clock(self):
+        return lib().nr_server_clock(self.h)
+
+    def submit(self, net, ks, kb, params, incs, keep_positions=False):
+        """Queues `net` for the strain increments incs ((n, 2) of xdisp, ydisp)."""
+        incs = np.ascontiguousarray(incs, np.float64).reshape(-1, 2)
+        j = C.c_int()
+        _check(lib().nr_server_submit(self.h, net.h, ks, kb, C.byref(params), len(incs),
+                                      _dp(incs), 1 if keep_positions else 0, C.byref(j)))
+        self.meta[j.value] = (net.info().n_nodes, len(incs), keep_positions)
+        return j.value
+
+    def step(self):
+        a, w = C.c_int(), C.c_int()
+        _check(lib().nr_server_step(self.h, C.byref(a), C.byref(w)))
+        return a.value, w.value
+
+    def drain(self):
+        _check(lib().nr_server_drain(self.h))
+
+    def results(self):
+        """Takes every finished job's result (list of dicts)."""
+        out = []
+        j = C.c_int()
+        while True:
+            rc = lib().nr_server_next_done(self.h, C.byref(j))
+            if rc == 1:
+                return out
+            _check(rc)
+            n, ninc, keep = self.meta.pop(j.value)
+            res = JobResult()
+            reps = (Report * ninc)()
+            pos = np.zeros(2 * n) if keep else None
+            _check(lib().nr_server_result(self.h, j.value, C.byref(res), reps,
+                                          _dp(pos) if keep else None))
+            d = {f: getattr(res, f) for f, _ in JobResult._fields_}
+            d["reports"] = [reps[i].as_dict() for i in range(res.n_inc)]
+            d["pos"] = pos
+            out.append(d)
+
+    def stats(self):
+        i = ServerInfo()
+        _check(lib().nr_server_stats(self.h, C.byref(i)))
+        return {f: getattr(i, f) for f, _ in ServerInfo._fields_}
 
 
 def relaxation_batch(batch_file, strain_mode=True, precision=PREC_FP64, tile_nodes=0):

@@ -11,15 +11,12 @@
 
 #include "../../include/nrelax.h"
 #include "device.hpp"
+#include "internal.hpp"
 #include "network.hpp"
 #include "plan.hpp"
 #include "teams.hpp"
 
-struct nr_network {
-    nr::Network net;
-};
-
-namespace {
+namespace nr {
 
 thread_local std::string g_err;
 
@@ -35,32 +32,31 @@ int hip_fail(hipError_t e, const char *what)
     return NR_E_HIP;
 }
 
-#define HIP_TRY(expr)                                  \
-    do {                                               \
-        hipError_t e_ = (expr);                        \
-        if (e_ != hipSuccess) return hip_fail(e_, #expr); \
-    } while (0)
+void to_report(const NetOut &o, nr_report &r)
+{
+    r.rms_force = o.rms;
+    r.max_force = o.fmax;
+    r.e_stretch = o.es;
+    r.e_bend = o.eb;
+    r.steps = o.steps;
+    r.status = o.status;
+    r.fp64_steps = o.fp64_steps;
+    r.pad = 0;
+    r.t_start = (double)o.t_start * 1e-8;
+    r.t_done = (double)o.t_done * 1e-8;
+}
 
-template <class T>
-struct DevBuf {
-    T *p = nullptr;
-    size_t n = 0;
-    ~DevBuf()
-    {
-        if (p) (void)hipFree(p);
-    }
-    hipError_t alloc(size_t count)
-    {
-        n = count;
-        return hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(count, 1));
-    }
-    hipError_t upload(const std::vector<T> &h)
-    {
-        hipError_t e = alloc(h.size());
-        if (e != hipSuccess || h.empty()) return e;
-        return hipMemcpy(p, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice);
-    }
-};
+}  // namespace nr
+
+namespace {
+
+using nr::DevBuf;
+using nr::fail;
+using nr::hip_fail;
+
+// Default tiles per team member: the best measured on MI355X for networks
+// of 1 to 150 tiles of 256 nodes.
+constexpr int kDefaultTilesPerMember = 4;
 
 // Device tables of one persistent launch.
 struct LaunchDev {
@@ -76,7 +72,7 @@ struct LaunchDev {
 struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
-    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 8;
+    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 4;
     size_t lds = 0;
     std::vector<std::unique_ptr<LaunchDev>> launches;
     std::vector<int> team_of_net;
@@ -93,6 +89,9 @@ struct nr_batch {
     DevBuf<nr::TileDev> tiles;
     DevBuf<nr::NetDev> dnet;
     DevBuf<nr::NetOut> dout;
+    DevBuf<nr::NetRun> run;        // resumable relax state, zeroed per call
+    DevBuf<double2> incs;          // strain ramp increments
+    DevBuf<nr::NetOut> inc_out;    // per-increment reports of a ramp
     DevBuf<unsigned> err;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -108,7 +107,7 @@ struct nr_batch {
 
 extern "C" {
 
-const char *nr_last_error(void) { return g_err.c_str(); }
+const char *nr_last_error(void) { return nr::g_err.c_str(); }
 
 const char *nr_version(void) { return "nrelax 0.1 (gfx950)"; }
 
@@ -255,22 +254,23 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     if (b->lds > 160 * 1024) return fail(NR_E_TOPO, "tile working set exceeds LDS");
 
     // teams: each network gets a power-of-two team of workgroups (a function
-    // of its tile count and the cap only); classes of equal team size share
-    // the resident workgroups in proportion to their work
+    // of its tile count and the options only, so its arithmetic does not
+    // depend on the rest of the batch); the resident workgroups are shared
+    // between the team-size classes to minimise the predicted finish time
     int per_cu = 0, n_cu = 0;
     HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
     if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
     const int resident = per_cu * n_cu;
     b->team_cap = opts && opts->team > 0 ? opts->team : 256;
-    b->tiles_per_member = opts && opts->tiles_per_member > 0 ? opts->tiles_per_member : 8;
+    b->tiles_per_member =
+        opts && opts->tiles_per_member > 0 ? opts->tiles_per_member : kDefaultTilesPerMember;
     if (b->team_cap > resident) b->team_cap = resident;
     {
-        std::vector<int> team(count), nodes(count);
-        for (int k = 0; k < count; k++) {
-            team[k] = nr::team_size(P.nets[k].n_tiles, b->team_cap, b->tiles_per_member);
-            nodes[k] = P.nets[k].n_nodes;
-        }
-        std::vector<nr::TeamLayout> layouts = nr::plan_launches(team, nodes, resident);
+        std::vector<nr::JobSpec> jobs(count);
+        for (int k = 0; k < count; k++)
+            jobs[k] = {nr::team_size(P.nets[k].n_tiles, b->team_cap, b->tiles_per_member),
+                       P.nets[k].n_tiles, nr::default_steps(P.nets[k].n_nodes)};
+        std::vector<nr::TeamLayout> layouts = nr::plan_launches(jobs, resident);
         b->team_of_net.assign(count, 1);
         for (nr::TeamLayout &L : layouts) {
             auto ld = std::make_unique<LaunchDev>();
@@ -345,6 +345,7 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     }
     HIP_TRY(b->dnet.upload(b->hnet));
     HIP_TRY(b->dout.alloc(count));
+    HIP_TRY(b->run.alloc(count));
     HIP_TRY(b->err.alloc(1));
     HIP_TRY(hipMemset(b->err.p, 0, sizeof(unsigned)));
     *out = b.release();
@@ -396,7 +397,13 @@ static int run_kernel(nr_batch *b, int mode)
     a.mode = mode;
     a.err = b->err.p;
     a.timeout_ticks = 100000000ll * 60;  // 60 s at the 100 MHz constant clock
+    a.run = b->run.p;
+    a.incs = b->incs.p;
+    a.inc_out = b->inc_out.p;
+    a.slice_steps = 0;
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    // every call starts fresh relaxes
+    HIP_TRY(hipMemsetAsync(b->run.p, 0, sizeof(nr::NetRun) * b->run.n, b->stream));
     for (auto &ld : b->launches) {
         HIP_TRY(hipMemsetAsync(ld->queues.p, 0, sizeof(int) * ld->queues.n, b->stream));
         a.roles = ld->roles.p;
@@ -426,9 +433,8 @@ static int run_kernel(nr_batch *b, int mode)
     return NR_OK;
 }
 
-int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared, nr_report *reports)
+static int set_params(nr_batch *b, const nr_relax_params *params, int shared)
 {
-    if (!b || !params) return fail(NR_E_ARG, "null argument");
     for (size_t k = 0; k < b->hnet.size(); k++) {
         const nr_relax_params &p = shared ? params[0] : params[k];
         if (p.method != NR_METHOD_FIRE) return fail(NR_E_ARG, "method not available");
@@ -447,24 +453,58 @@ int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared, nr_re
         d.crit = p.criterion;
         d.method = p.method;
     }
-    int rc = run_kernel(b, nr::kModeRelax);
+    return NR_OK;
+}
+
+int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared, nr_report *reports)
+{
+    if (!b || !params) return fail(NR_E_ARG, "null argument");
+    int rc = set_params(b, params, shared);
+    if (rc) return rc;
+    rc = run_kernel(b, nr::kModeRelax);
     if (rc) return rc;
     if (reports) {
         std::vector<nr::NetOut> o(b->hnet.size());
         HIP_TRY(hipMemcpy(o.data(), b->dout.p, sizeof(nr::NetOut) * o.size(),
                           hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < o.size(); k++) {
-            reports[k].rms_force = o[k].rms;
-            reports[k].max_force = o[k].fmax;
-            reports[k].e_stretch = o[k].es;
-            reports[k].e_bend = o[k].eb;
-            reports[k].steps = o[k].steps;
-            reports[k].status = o[k].status;
-            reports[k].fp64_steps = o[k].fp64_steps;
-            reports[k].pad = 0;
-            reports[k].t_start = (double)o[k].t_start * 1e-8;
-            reports[k].t_done = (double)o[k].t_done * 1e-8;
-        }
+        for (size_t k = 0; k < o.size(); k++) nr::to_report(o[k], reports[k]);
+    }
+    return NR_OK;
+}
+
+int nr_batch_ramp(nr_batch *b, int n_inc, const double *incs, const nr_relax_params *params,
+                  int shared, nr_report *reports)
+{
+    if (!b || !params || (n_inc > 0 && !incs) || n_inc < 0)
+        return fail(NR_E_ARG, "invalid arguments");
+    if (n_inc == 0) return NR_OK;
+    const int count = (int)b->hnet.size();
+    for (int i = 0; i < n_inc; i++) {
+        if (!std::isfinite(incs[2 * i]) || !std::isfinite(incs[2 * i + 1]))
+            return fail(NR_E_ARG, "non-finite strain increment");
+        for (int k = 0; k < count; k++)
+            if (b->hnet[k].height > 0 && incs[2 * i + 1] != 0.0)
+                return fail(NR_E_ARG, "compression is not defined in the Lees-Edwards box");
+    }
+    int rc = set_params(b, params, shared);
+    if (rc) return rc;
+    std::vector<double2> h(n_inc);
+    for (int i = 0; i < n_inc; i++) h[i] = make_double2(incs[2 * i], incs[2 * i + 1]);
+    HIP_TRY(b->incs.upload(h));
+    if (b->inc_out.n < (size_t)count * n_inc) HIP_TRY(b->inc_out.alloc((size_t)count * n_inc));
+    for (int k = 0; k < count; k++) {
+        b->hnet[k].inc_count = n_inc;
+        b->hnet[k].inc_base = 0;
+        b->hnet[k].out_base = k * n_inc;
+    }
+    rc = run_kernel(b, nr::kModeRelax);
+    for (int k = 0; k < count; k++) b->hnet[k].inc_count = 0;
+    if (rc) return rc;
+    if (reports) {
+        std::vector<nr::NetOut> o((size_t)count * n_inc);
+        HIP_TRY(hipMemcpy(o.data(), b->inc_out.p, sizeof(nr::NetOut) * o.size(),
+                          hipMemcpyDeviceToHost));
+        for (size_t i = 0; i < o.size(); i++) nr::to_report(o[i], reports[i]);
     }
     return NR_OK;
 }

@@ -15,11 +15,32 @@ struct NetDev {
     int node_base, n_nodes, tile_base, n_tiles;
     int parity;       // which state buffer holds the current x, v, F
     int crit, nmin, max_steps;
-    int method, pad0, pad1, pad2;
+    int method;
+    int inc_count, inc_base;  // strain increments of this network in KArgs::incs
+    int out_base;             // its per-increment reports in KArgs::inc_out
     double width, height, le_shift, miny, yrange;
     double ks, kb;
     double a0, fa, dtmax, finc, fdec, mass, fcut;
     double pend_x, pend_y;  // affine step applied by the next launch
+};
+
+// Resumable relax state of one network (all zero = start a fresh relax).
+// A launch with a step budget stops a relax mid-way and saves here what the
+// next launch needs to continue it with the same arithmetic.
+enum RunPhase { kRunFresh = 0, kRun32 = 1, kRun64 = 2 };
+struct NetRun {
+    int phase;
+    int step, steps32;       // FIRE steps of the current increment, FP32 stage
+    int since;               // FIRE steps since the power was last negative
+    int since_best;          // FP32 stage: steps without a new minimum
+    int mix, s;              // deferred mix of the next pass; FP32 parity
+    int inc_next;            // increments of the list already started
+    double alpha, dt;        // FIRE mixing factor and time step
+    double alpha_old, cmix;  // deferred mix coefficients
+    double best, fswitch;    // FP32 stage: least max |F|^2, hand-over threshold
+    double ff, fmax2;        // |F|^2 and max node |F|^2 after the last pass
+    long long t_start;       // device clock when the increment began
+    long long pad;
 };
 
 struct NetOut {
@@ -63,6 +84,10 @@ struct KArgs {
     int *mailbox;             // per team, stride 32
     unsigned *err;            // nonzero after a bounded wait expired
     long long timeout_ticks;
+    NetRun *run;              // per network resumable state
+    const double2 *incs;      // strain increments (xdisp, ydisp)
+    NetOut *inc_out;          // one report per increment, at NetDev::out_base
+    int slice_steps;          // FIRE steps per network per launch, 0 = no limit
 };
 
 // Team barrier words per team: [0] root arrivals, [16] generation,

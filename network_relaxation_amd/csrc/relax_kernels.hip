@@ -459,24 +459,22 @@ __device__ inline double2 gather_force(const Blob &bb, const double2 *fv, int k)
 // :679-712): the mix of the step just taken is deferred to the next pass's
 // position update (S.alpha_old, S.cmix, S.mix).
 struct Fire {
-    double a0, fa, dtmax, finc, fdec, mass;
-    int nmin;
-    double alpha, dt, dtm, dtsq;
+    const NetDev *nd;  // FIRE constants, read where used (scalar cache)
+    double alpha, dt;
     int since;
 
     __device__ void start()
     {
-        alpha = a0;
-        dt = dtmax;
-        dtm = 0.5 * dt / mass;
-        dtsq = 0.5 * dt * dt / mass;
+        alpha = nd->a0;
+        dt = nd->dtmax;
         since = 0;
     }
     __device__ void load(StepScal &S) const
     {
+        const double mass = nd->mass;
         S.dt = dt;
-        S.dtsq = dtsq;
-        S.dtm = dtm;
+        S.dtsq = 0.5 * dt * dt / mass;
+        S.dtm = 0.5 * dt / mass;
     }
     __device__ void update(const Red4 &r, StepScal &S)
     {
@@ -485,19 +483,15 @@ struct Fire {
         S.cmix = alpha * vmag / fmag;
         if (P > 0) {  // :694-703
             since++;
-            if (since > nmin) {
-                dt = fmin(dt * finc, dtmax);
-                dtm = 0.5 * dt / mass;
-                dtsq = 0.5 * dt * dt / mass;
-                alpha *= fa;
+            if (since > nd->nmin) {
+                dt = fmin(dt * nd->finc, nd->dtmax);
+                alpha *= nd->fa;
             }
             S.mix = kMixFire;
         } else {  // :704-712
             since = 0;
-            dt *= fdec;
-            dtm = 0.5 * dt / mass;
-            dtsq = 0.5 * dt * dt / mass;
-            alpha = a0;
+            dt *= nd->fdec;
+            alpha = nd->a0;
             S.mix = kMixZero;
         }
     }
@@ -917,7 +911,8 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                               int *s_flag)
 {
     NetDev *nd = A.nets + net_id;
-    const long long t_start = wall_clock64();
+    NetRun *rs = A.run + net_id;
+    const bool lead = T.member == 0 && threadIdx.x == 0;  // writes the job's records
     NetCtx C;
     C.node_base = nd->node_base;
     C.n_nodes = nd->n_nodes;
@@ -929,14 +924,13 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     C.yrange = nd->yrange;
     C.ks = nd->ks;
     C.kb = nd->kb;
-    const double pend_x = nd->pend_x, pend_y = nd->pend_y;
-    // the strain step advances the Lees-Edwards image shift before the pass
-    C.shift = nd->le_shift + (C.H > 0 ? pend_x : 0.0);
+    double shift = nd->le_shift;
     int p = nd->parity;
-    const double a0 = nd->a0, fa = nd->fa, dtmax = nd->dtmax, finc = nd->finc, fdec = nd->fdec,
-                 mass = nd->mass, fcut = nd->fcut;
-    const int nmin = nd->nmin, max_steps = A.mode == kModeEval ? 0 : nd->max_steps;
-    const int crit = nd->crit;
+    const int max_steps = A.mode == kModeEval ? 0 : nd->max_steps;
+    const int budget = A.slice_steps > 0 ? A.slice_steps : 0x7fffffff;
+    int used = 0;
+    // resumable state kept in registers; the rest of NetRun is read where needed
+    int phase = rs->phase, inc_next = rs->inc_next;
 
     // a member owning exactly one tile keeps that tile's blob in LDS
     const bool resident = T.member < C.n_tiles && C.n_tiles <= T.member + T.g;
@@ -945,127 +939,238 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
         __syncthreads();
     }
 
+    Fire fire;
+    fire.nd = nd;
     StepScal S;
     S.dt = 0.0;
     S.dtsq = 0.0;
     S.dtm = 0.0;
-    S.alpha_old = 0.0;
-    S.cmix = 0.0;
-    S.mix = kMixNone;
-    S.pend_x = pend_x;
-    S.pend_y = pend_y;
-    S.s = 0;
-
-    int status = NR_NOT_CONVERGED;
+    S.pend_x = 0.0;
+    S.pend_y = 0.0;
     Red4 r = {0.0, 0.0, 0.0, 0.0};
     Red4 part;
-    const double sqrt_dof = sqrt((double)(2 * C.n_nodes));
-    Fire fire;
-    fire.a0 = a0;
-    fire.fa = fa;
-    fire.dtmax = dtmax;
-    fire.finc = finc;
-    fire.fdec = fdec;
-    fire.mass = mass;
-    fire.nmin = nmin;
-    int step = 0, steps32 = 0;
+    int status = NR_NOT_CONVERGED;
+    bool paused = false;
+    bool host_step_used = false;
+    int step, steps32;
+    double ff, fmax2;
 
-    if (A.mixed && max_steps > 0) {
-        // FP32 stage: anchor = positions after the strain step, in x[p ^ 1]
-        const int pa = p ^ 1;
-        S.s = 0;
-        part = pass32<B, kPass32Init>(A, T, C, S, p, pa, L, resident);
-        if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
-        S.s = 1;
-        const double fswitch = fmax(fcut, A.switch_ratio * sqrt(r.m));
-        double best = r.m;
-        int since_best = 0;
-        fire.start();
-        S.mix = kMixNone;
-        while (status != NR_ABORTED && step < max_steps) {
-            step++;
-            fire.load(S);
-            part = pass32<B, kPass32Step>(A, T, C, S, p, pa, L, resident);
-            if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
-                status = NR_ABORTED;
-                break;
+    for (;;) {  // strain increments
+        const bool fresh = phase == kRunFresh;
+        if (fresh) {
+            // strain step set by the host (first increment of this launch
+            // only; the record is cleared when the job's visit ends, after
+            // every member has read it) plus the next increment of the list
+            double pend_x = 0.0, pend_y = 0.0;
+            if (!host_step_used) {
+                pend_x = nd->pend_x;
+                pend_y = nd->pend_y;
+                host_step_used = true;
             }
-            S.s ^= 1;
-            fire.update(r, S);
-            if (!isfinite(r.c) || sqrt(r.m) < fswitch) break;
-            if (r.m < best) {
+            if (inc_next < nd->inc_count) {
+                const double2 d = A.incs[nd->inc_base + inc_next];
+                pend_x += d.x;
+                pend_y += d.y;
+                inc_next++;
+            }
+            // the strain step advances the Lees-Edwards image shift
+            C.shift = shift + (C.H > 0 ? pend_x : 0.0);
+            S.pend_x = pend_x;
+            S.pend_y = pend_y;
+            S.alpha_old = 0.0;
+            S.cmix = 0.0;
+            S.mix = kMixNone;
+            S.s = 0;
+            if (lead) rs->t_start = wall_clock64();
+            step = 0;
+            steps32 = 0;
+            fire.start();
+            phase = A.mixed && max_steps > 0 ? kRun32 : kRun64;
+        } else {
+            C.shift = shift;
+            step = rs->step;
+            steps32 = rs->steps32;
+            ff = rs->ff;
+            fmax2 = rs->fmax2;
+            fire.alpha = rs->alpha;
+            fire.dt = rs->dt;
+            fire.since = rs->since;
+            S.alpha_old = rs->alpha_old;
+            S.cmix = rs->cmix;
+            S.mix = rs->mix;
+            S.s = rs->s;
+        }
+
+        if (phase == kRun32) {
+            // FP32 stage: anchor = positions after the strain step, in x[p ^ 1]
+            const int pa = p ^ 1;
+            double best, fswitch;
+            int since_best;
+            if (fresh) {
+                part = pass32<B, kPass32Init>(A, T, C, S, p, pa, L, resident);
+                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+                S.s = 1;
+                S.pend_x = 0.0;
+                S.pend_y = 0.0;
+                fswitch = fmax(nd->fcut, A.switch_ratio * sqrt(r.m));
                 best = r.m;
                 since_best = 0;
-            } else if (++since_best > A.stall_steps) {
-                break;
+            } else {
+                fswitch = rs->fswitch;
+                best = rs->best;
+                since_best = rs->since_best;
+            }
+            while (status != NR_ABORTED && step < max_steps) {
+                if (used >= budget) {
+                    paused = true;
+                    break;
+                }
+                step++;
+                used++;
+                fire.load(S);
+                part = pass32<B, kPass32Step>(A, T, C, S, p, pa, L, resident);
+                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
+                    status = NR_ABORTED;
+                    break;
+                }
+                S.s ^= 1;
+                fire.update(r, S);
+                if (!isfinite(r.c) || sqrt(r.m) < fswitch) break;
+                if (r.m < best) {
+                    best = r.m;
+                    since_best = 0;
+                } else if (++since_best > A.stall_steps) {
+                    break;
+                }
+            }
+            if (paused) {
+                if (lead) {
+                    rs->fswitch = fswitch;
+                    rs->best = best;
+                    rs->since_best = since_best;
+                }
+            } else if (status != NR_ABORTED) {
+                // FP64 polish from anchor + displacement; FIRE continues with
+                // the velocity carried over by the fuse
+                steps32 = step;
+                p = pa;
+                part = pass<B, NPT, kPassFuse>(A, T, C, S, p, L, resident);
+                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+                p ^= 1;
+                ff = r.c;
+                fmax2 = r.m;
+                phase = kRun64;
+                if (steps32 == 0) {
+                    fire.start();
+                    S.mix = kMixNone;
+                }
+            }
+        } else if (fresh) {
+            part = pass<B, NPT, kPassInit>(A, T, C, S, p, L, resident);
+            if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+            S.pend_x = 0.0;
+            S.pend_y = 0.0;
+            p ^= 1;
+            ff = r.c;
+            fmax2 = r.m;
+        }
+        shift = C.shift;
+
+        if (phase == kRun64 && !paused && status != NR_ABORTED) {
+            // run_fire_md :624-718
+            const double fcut = nd->fcut;
+            const int crit = nd->crit;
+            const double sqrt_dof = sqrt((double)(2 * C.n_nodes));
+            while (step < max_steps) {
+                if (used >= budget) {
+                    paused = true;
+                    break;
+                }
+                step++;
+                used++;
+                fire.load(S);
+                part = pass<B, NPT, kPassStep>(A, T, C, S, p, L, resident);
+                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
+                    status = NR_ABORTED;
+                    break;
+                }
+                p ^= 1;
+                ff = r.c;
+                fmax2 = r.m;
+                fire.update(r, S);
+                const bool conv =
+                    crit == NR_CRIT_MAX ? sqrt(fmax2) < fcut : sqrt(ff) / sqrt_dof < fcut;
+                if (conv) break;
+                if (!isfinite(ff)) break;
             }
         }
-        steps32 = step;
-        // FP64 polish from anchor + displacement
-        p = pa;
-        S.pend_x = 0.0;
-        S.pend_y = 0.0;
-        if (status != NR_ABORTED) part = pass<B, NPT, kPassFuse>(A, T, C, S, p, L, resident);
-    } else {
-        part = pass<B, NPT, kPassInit>(A, T, C, S, p, L, resident);
-    }
-    if (status != NR_ABORTED && !reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
-    p ^= 1;
-    double ff = r.c, fmax2 = r.m;
-    double fmag = sqrt(ff);
-
-    // run_fire_md :616-622 (a mixed-precision relax continues the FP32
-    // stage's FIRE state, its velocity having been carried over by the fuse)
-    if (!(A.mixed && steps32 > 0)) {
-        fire.start();
-        S.mix = kMixNone;
-    }
-    while (status != NR_ABORTED && step < max_steps) {
-        step++;
-        fire.load(S);
-        part = pass<B, NPT, kPassStep>(A, T, C, S, p, L, resident);
-        if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
-            status = NR_ABORTED;
+        if (paused) {
+            status = NR_IN_PROGRESS;
             break;
         }
-        p ^= 1;
-        ff = r.c;
-        fmax2 = r.m;
-        fmag = sqrt(ff);
-        fire.update(r, S);
-        const bool conv = crit == NR_CRIT_MAX ? sqrt(fmax2) < fcut : fmag / sqrt_dof < fcut;
-        if (conv) break;
-        if (!isfinite(ff)) break;
-    }
-    if (status != NR_ABORTED) {
-        if (!isfinite(ff)) {
-            status = NR_NONFINITE;
-        } else {
-            const double meas = crit == NR_CRIT_MAX ? sqrt(fmax2) : fmag / sqrt_dof;
-            status = meas < fcut ? NR_CONVERGED : NR_NOT_CONVERGED;
+        const double sqrt_dof = sqrt((double)(2 * C.n_nodes));
+        if (status != NR_ABORTED) {
+            if (!isfinite(ff)) {
+                status = NR_NONFINITE;
+            } else {
+                const double meas = nd->crit == NR_CRIT_MAX ? sqrt(fmax2) : sqrt(ff) / sqrt_dof;
+                status = meas < nd->fcut ? NR_CONVERGED : NR_NOT_CONVERGED;
+            }
         }
+        // energies at the final positions (:723-757)
+        Red4 e = {0.0, 0.0, 0.0, 0.0};
+        if (status != NR_ABORTED) {
+            part = pass<B, NPT, kPassEnergy>(A, T, C, S, p, L, resident);
+            if (!reduce4<B>(A, T, part, e, s_red, s_flag)) status = NR_ABORTED;
+        }
+        if (lead) {
+            NetOut o;
+            o.rms = sqrt(ff) / sqrt_dof;
+            o.fmax = sqrt(fmax2);
+            o.es = C.ks > 0 ? e.a : 0.0;
+            o.eb = C.kb > 0 ? e.b : 0.0;
+            o.steps = step;
+            o.status = status;
+            o.fp64_steps = step - steps32;
+            o.pad = 0;
+            o.t_start = rs->t_start;
+            o.t_done = wall_clock64();
+            A.out[net_id] = o;
+            if (inc_next > 0 && inc_next <= nd->inc_count) A.inc_out[nd->out_base + inc_next - 1] = o;
+        }
+        phase = kRunFresh;
+        // the next increment of the list starts from these positions
+        if ((status == NR_CONVERGED || status == NR_NOT_CONVERGED) && inc_next < nd->inc_count)
+            continue;
+        break;
     }
-    // energies at the final positions (:723-757)
-    Red4 e = {0.0, 0.0, 0.0, 0.0};
-    if (status != NR_ABORTED) {
-        part = pass<B, NPT, kPassEnergy>(A, T, C, S, p, L, resident);
-        if (!reduce4<B>(A, T, part, e, s_red, s_flag)) status = NR_ABORTED;
-    }
-    if (T.member == 0 && threadIdx.x == 0) {
-        NetOut o;
-        o.rms = fmag / sqrt_dof;
-        o.fmax = sqrt(fmax2);
-        o.es = C.ks > 0 ? e.a : 0.0;
-        o.eb = C.kb > 0 ? e.b : 0.0;
-        o.steps = step;
-        o.status = status;
-        o.fp64_steps = step - steps32;
-        o.pad = 0;
-        o.t_start = t_start;
-        o.t_done = wall_clock64();
-        A.out[net_id] = o;
+    if (lead) {
+        if (paused) {
+            rs->step = step;
+            rs->steps32 = steps32;
+            rs->ff = ff;
+            rs->fmax2 = fmax2;
+            rs->alpha = fire.alpha;
+            rs->dt = fire.dt;
+            rs->since = fire.since;
+            rs->alpha_old = S.alpha_old;
+            rs->cmix = S.cmix;
+            rs->mix = S.mix;
+            rs->s = S.s;
+            NetOut o = {};
+            o.rms = sqrt(ff) / sqrt((double)(2 * C.n_nodes));
+            o.fmax = sqrt(fmax2);
+            o.steps = step;
+            o.status = NR_IN_PROGRESS;
+            o.fp64_steps = step - steps32;
+            o.t_start = rs->t_start;
+            o.t_done = wall_clock64();
+            A.out[net_id] = o;
+        }
+        rs->phase = phase;
+        rs->inc_next = inc_next;
         nd->parity = p;
-        nd->le_shift = C.shift;
+        nd->le_shift = shift;
         nd->pend_x = 0.0;
         nd->pend_y = 0.0;
     }

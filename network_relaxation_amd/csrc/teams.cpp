@@ -3,9 +3,10 @@
 #include "teams.hpp"
 
 #include <algorithm>
-#include <cmath>
 #include <functional>
+#include <limits>
 #include <map>
+#include <queue>
 
 namespace nr {
 
@@ -18,72 +19,98 @@ int team_size(int n_tiles, int cap, int tiles_per_member)
     return g;
 }
 
-TeamLayout plan_teams(const std::vector<int> &team, const std::vector<int> &n_nodes,
-                      int resident)
+double job_cost(double steps, int n_tiles, int g)
+{
+    const double passes = (double)((n_tiles + g - 1) / g);
+    return steps * (passes + (g > 1 ? 0.25 : 0.0));
+}
+
+namespace {
+
+// Finish time of `teams` identical teams serving `costs` (sorted descending)
+// longest first, each job going to the team that frees up first -- the order
+// in which the device queue hands them out.
+double lpt_finish(const std::vector<double> &costs, int teams)
+{
+    std::priority_queue<double, std::vector<double>, std::greater<double>> free_at;
+    for (int t = 0; t < teams; t++) free_at.push(0.0);
+    double end = 0.0;
+    for (double c : costs) {
+        double t0 = free_at.top();
+        free_at.pop();
+        end = std::max(end, t0 + c);
+        free_at.push(t0 + c);
+    }
+    return end;
+}
+
+}  // namespace
+
+TeamLayout plan_teams(const std::vector<JobSpec> &jobs, int resident)
 {
     TeamLayout T;
-    const int count = (int)team.size();
-    T.team_of_net = team;
-    std::map<int, std::vector<int>> classes;  // team size -> networks
-    for (int k = 0; k < count; k++) classes[team[k]].push_back(k);
-    std::vector<int> gs;
-    std::vector<double> work;
-    std::vector<int> cnt;
-    double total = 0.0;
-    for (auto &kv : classes) {
-        double w = 0.0;
-        for (int k : kv.second) w += std::pow((double)n_nodes[k], 1.5);
-        gs.push_back(kv.first);
-        work.push_back(w);
-        cnt.push_back((int)kv.second.size());
-        total += w;
+    const int count = (int)jobs.size();
+    T.team_of_net.resize(count);
+    std::map<int, std::vector<int>> classes;  // team size -> jobs
+    for (int k = 0; k < count; k++) {
+        classes[jobs[k].team].push_back(k);
+        T.team_of_net[k] = jobs[k].team;
     }
-    const int nc = (int)gs.size();
-    // teams per class: at least one, at most one per network, blocks ~ work
-    std::vector<int> teams(nc, 1);
-    int used = 0;
-    for (int c = 0; c < nc; c++) used += gs[c];
+    const int nc = (int)classes.size();
+    std::vector<int> gs, teams(nc, 1);
+    std::vector<std::vector<int>> members;
+    std::vector<std::vector<double>> costs(nc);
+    std::vector<double> finish(nc);
+    int used = 0, c = 0;
+    for (auto &kv : classes) {
+        std::vector<int> js = kv.second;
+        const int g = kv.first;
+        // longest expected relax first: the queue order on the device
+        std::stable_sort(js.begin(), js.end(), [&](int a, int b) {
+            return job_cost(jobs[a].steps, jobs[a].n_tiles, g) >
+                   job_cost(jobs[b].steps, jobs[b].n_tiles, g);
+        });
+        for (int k : js) costs[c].push_back(job_cost(jobs[k].steps, jobs[k].n_tiles, g));
+        finish[c] = lpt_finish(costs[c], 1);
+        gs.push_back(g);
+        members.push_back(js);
+        used += g;
+        c++;
+    }
+    // one more team at a time to the latest-finishing class that can use it
     while (true) {
         int best = -1;
-        double best_deficit = 0.0;
-        for (int c = 0; c < nc; c++) {
-            if (teams[c] >= cnt[c] || used + gs[c] > resident) continue;
-            double share = work[c] / total * resident;
-            double deficit = share - teams[c] * gs[c];
-            if (deficit > best_deficit) {
-                best_deficit = deficit;
-                best = c;
-            }
+        for (int i = 0; i < nc; i++) {
+            if (teams[i] >= (int)costs[i].size() || used + gs[i] > resident) continue;
+            if (best < 0 || finish[i] > finish[best]) best = i;
         }
         if (best < 0) break;
         teams[best]++;
         used += gs[best];
+        finish[best] = lpt_finish(costs[best], teams[best]);
     }
+    T.makespan = 0.0;
+    for (int i = 0; i < nc; i++) T.makespan = std::max(T.makespan, finish[i]);
     // roles, job lists
     int team_id = 0, part = 0;
     T.class_off.push_back(0);
-    int c = 0;
-    for (auto &kv : classes) {
-        std::vector<int> jobs = kv.second;
-        std::stable_sort(jobs.begin(), jobs.end(),
-                         [&](int a, int b) { return n_nodes[a] > n_nodes[b]; });
-        T.class_jobs.insert(T.class_jobs.end(), jobs.begin(), jobs.end());
+    for (int i = 0; i < nc; i++) {
+        T.class_jobs.insert(T.class_jobs.end(), members[i].begin(), members[i].end());
         T.class_off.push_back((int)T.class_jobs.size());
-        for (int t = 0; t < teams[c]; t++) {
-            for (int m = 0; m < gs[c]; m++) {
+        for (int t = 0; t < teams[i]; t++) {
+            for (int m = 0; m < gs[i]; m++) {
                 BlockRole r = {};
                 r.team = team_id;
                 r.member = m;
-                r.g = gs[c];
-                r.cls = c;
+                r.g = gs[i];
+                r.cls = i;
                 r.part = part;
                 T.roles.push_back(r);
             }
             team_id++;
-            part += gs[c];
-            if (gs[c] > 1) T.cooperative = true;
+            part += gs[i];
+            if (gs[i] > 1) T.cooperative = true;
         }
-        c++;
     }
     T.grid = (int)T.roles.size();
     T.n_teams = team_id;
@@ -91,12 +118,11 @@ TeamLayout plan_teams(const std::vector<int> &team, const std::vector<int> &n_no
     return T;
 }
 
-std::vector<TeamLayout> plan_launches(const std::vector<int> &team,
-                                      const std::vector<int> &n_nodes, int resident)
+std::vector<TeamLayout> plan_launches(const std::vector<JobSpec> &jobs, int resident)
 {
-    const int count = (int)team.size();
+    const int count = (int)jobs.size();
     std::map<int, std::vector<int>, std::greater<int>> classes;
-    for (int k = 0; k < count; k++) classes[team[k]].push_back(k);
+    for (int k = 0; k < count; k++) classes[jobs[k].team].push_back(k);
     // first-fit of classes (largest team first) into launches
     std::vector<std::vector<int>> groups;
     std::vector<int> room;
@@ -118,12 +144,9 @@ std::vector<TeamLayout> plan_launches(const std::vector<int> &team,
     std::vector<TeamLayout> out;
     for (auto &grp : groups) {
         std::sort(grp.begin(), grp.end());
-        std::vector<int> t, n;
-        for (int k : grp) {
-            t.push_back(team[k]);
-            n.push_back(n_nodes[k]);
-        }
-        TeamLayout L = plan_teams(t, n, resident);
+        std::vector<JobSpec> sub;
+        for (int k : grp) sub.push_back(jobs[k]);
+        TeamLayout L = plan_teams(sub, resident);
         for (int &j : L.class_jobs) j = grp[j];
         std::vector<int> tm(count, 0);
         for (size_t i = 0; i < grp.size(); i++) tm[grp[i]] = L.team_of_net[i];
@@ -131,6 +154,27 @@ std::vector<TeamLayout> plan_launches(const std::vector<int> &team,
         out.push_back(std::move(L));
     }
     return out;
+}
+
+int tune_tiles_per_member(const std::vector<int> &n_tiles, const std::vector<double> &steps,
+                          int cap, int resident, const std::vector<int> &candidates,
+                          double *predicted)
+{
+    int best = candidates.empty() ? 1 : candidates[0];
+    double best_t = std::numeric_limits<double>::infinity();
+    for (int r : candidates) {
+        std::vector<JobSpec> jobs(n_tiles.size());
+        for (size_t k = 0; k < n_tiles.size(); k++)
+            jobs[k] = {team_size(n_tiles[k], cap, r), n_tiles[k], steps[k]};
+        double t = 0.0;
+        for (const TeamLayout &L : plan_launches(jobs, resident)) t += L.makespan;
+        if (t < best_t) {
+            best_t = t;
+            best = r;
+        }
+    }
+    if (predicted) *predicted = best_t;
+    return best;
 }
 
 }  // namespace nr

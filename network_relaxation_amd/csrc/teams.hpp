@@ -1,6 +1,7 @@
 // Team layout of the persistent relax grid.
 #pragma once
 
+#include <cmath>
 #include <cstdint>
 #include <vector>
 
@@ -13,6 +14,7 @@ struct TeamLayout {
     int n_teams = 0;
     int n_parts = 0;                // partial-sum slots (sum of team sizes)
     bool cooperative = false;       // some team has more than one member
+    double makespan = 0.0;          // predicted, in cost-model units
     std::vector<BlockRole> roles;   // per workgroup
     std::vector<int> class_jobs;    // networks per class, largest first
     std::vector<int> class_off;     // [n_classes + 1]
@@ -25,17 +27,35 @@ struct TeamLayout {
 // alone and inside any batch.
 int team_size(int n_tiles, int cap, int tiles_per_member);
 
-// Splits `resident` workgroups between the team-size classes (team[k] is
-// network k's team size) present in the
-// batch in proportion to their estimated work (nodes^1.5 per network), each
-// class getting at least one team and at most one team per network.
-TeamLayout plan_teams(const std::vector<int> &team, const std::vector<int> &n_nodes,
-                      int resident);
+// Cost model of one relax, fitted to measured batch runs on MI355X: a FIRE
+// step of a network costs the tile passes of its busiest member plus a team
+// barrier worth a quarter of a tile pass, and the number of steps grows like
+// the lattice side.  `steps` is the expected step count (any unit).
+double job_cost(double steps, int n_tiles, int g);
+// Expected steps of a network with no history: the lattice side.
+inline double default_steps(int n_nodes) { return std::sqrt((double)n_nodes); }
+
+// One job of a launch plan: its team size, tile count and expected steps.
+struct JobSpec {
+    int team, n_tiles;
+    double steps;
+};
+
+// Splits `resident` workgroups between the team-size classes present in the
+// jobs: every class gets one team, then teams go one at a time to the class
+// whose predicted finish (longest-processing-time-first over its teams) is
+// latest, while they fit.  A class never gets more teams than jobs.
+TeamLayout plan_teams(const std::vector<JobSpec> &jobs, int resident);
 
 // Like plan_teams, but when the classes cannot all be co-resident (their
 // team sizes add up to more than `resident`) the classes are split over
 // several launches, largest teams first.  Job lists hold batch indices.
-std::vector<TeamLayout> plan_launches(const std::vector<int> &team,
-                                      const std::vector<int> &n_nodes, int resident);
+std::vector<TeamLayout> plan_launches(const std::vector<JobSpec> &jobs, int resident);
+
+// Tiles per member that minimises the predicted time of the job set over
+// `candidates` (plan_launches makespans added over launches).
+int tune_tiles_per_member(const std::vector<int> &n_tiles, const std::vector<double> &steps,
+                          int cap, int resident, const std::vector<int> &candidates,
+                          double *predicted);
 
 }  // namespace nr
