@@ -4,12 +4,16 @@ Workload (synthetic, seeded): 512 independent Lees-Edwards triangular networks,
 L uniform even in [16, 192], bond dilution p ~ U[0.55, 0.75], per-fiber bending
 stiffness log-uniform in [1e-5, 1e-2], each sheared affinely to gamma ~ U[0, 0.4]
 and relaxed by FIRE to max node |F| < 1e-9.  A step = relax the whole batch
-from the affine start (positions restored on the device before each step).
+from the affine start (positions restored on the device before each step) in
+mixed precision (FP32 stage on FP64 anchors, FP64 polish); the headline value
+is that form's converged networks/s.
 
-Also measured: the plain form (one network per launch, FP64, the reference's
-way of running) on a sample of the same networks, the relax kernel's algorithmic
-HBM rate against MI355X's 8 TB/s, and the CPU FP64 oracle on the host cores
-over a bounded sample.  Prints one JSON line.
+Also measured, each on the same MI355X: the same batch in FP64 (and the
+per-network energy difference of the mixed form from it), the plain form (one
+network per launch, FP64) on a sample, a served stream (Poisson arrivals of
+strain-ramp jobs through the serving loop, p50/p99 latency), the relax
+kernel's algorithmic HBM rate against MI355X's 8 TB/s, and the CPU FP64 oracle
+on the host cores over a bounded sample.  Prints one JSON line.
 """
 import argparse
 import json
@@ -21,6 +25,12 @@ from concurrent.futures import ThreadPoolExecutor
 import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
+T_START = time.perf_counter()
+
+
+def progress(msg):
+    """One progress line on stderr (the JSON result goes to stdout)."""
+    print(f"[bench {time.perf_counter() - T_START:7.1f}s] {msg}", file=sys.stderr, flush=True)
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
@@ -55,6 +65,13 @@ def alg_bytes(N_, E, T, per_triple_kappa=True):
     return 97 * N_ + 17 * E + (22 if per_triple_kappa else 14) * T
 
 
+def alg_bytes32(N_, E, T):
+    """Algorithmic bytes of one FP32-stage step: FP64 anchor read (16 B/node),
+    FP32 u, v, F read+written (48), mask (1); bond with an FP32 rest-length
+    term (13 B); triple (14) and its f64 kappa (8)."""
+    return 65 * N_ + 13 * E + 22 * T
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -68,10 +85,13 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--team", type=int, default=0)
     ap.add_argument("--tpm", type=int, default=0, help="tiles per team member")
+    ap.add_argument("--switch", type=float, default=0.0, help="mixed FP32 -> FP64 hand-over")
     ap.add_argument("--plain-sample", type=int, default=8)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--served", type=int, default=64, help="served requests (0 = skip)")
+    ap.add_argument("--served-lmax", type=int, default=128)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
@@ -89,40 +109,64 @@ def main():
 
     specs = workload(args.networks, args.lmin, args.lmax, args.seed)
     nets = build_networks(N, specs)
-    batch = N.Batch(nets, 1.0, 1.0, tile_nodes=args.tile, team=args.team,
-                    tiles_per_member=args.tpm)
+    progress(f"built {len(nets)} networks")
     prm = N.relax_params(fcut=args.fcut, max_steps=args.max_steps, criterion=N.CRIT_MAX, **FIRE)
+    shape = dict(tile_nodes=args.tile, team=args.team, tiles_per_member=args.tpm)
 
-    def one_step():
-        batch.reset()
-        for k, s in enumerate(specs):
-            batch.shear(k, s["gamma"])
-        return batch.relax(prm)
+    def run_batch(batch, steps, warmup):
+        def one_step():
+            batch.reset()
+            for k, s in enumerate(specs):
+                batch.shear(k, s["gamma"])
+            return batch.relax(prm)
+        for w in range(warmup):
+            one_step()
+            progress(f"warmup step {w + 1}/{warmup}: {batch.last_ms():.0f} ms")
+        kernel_ms = 0.0
+        t0 = time.perf_counter()
+        for k in range(steps):
+            reps = one_step()
+            kernel_ms += batch.last_ms()
+            progress(f"timed step {k + 1}/{steps}: {batch.last_ms():.0f} ms")
+        return reps, time.perf_counter() - t0, kernel_ms / steps
 
-    for _ in range(args.warmup):
-        one_step()
-    kernel_ms = 0.0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        reps = one_step()
-        kernel_ms += batch.last_ms()
-    wall = time.perf_counter() - t0
-
-    conv = [r["status"] == N.CONVERGED for r in reps]
-    n_conv = int(sum(conv))
+    # headline: batched + mixed precision
+    mixed = N.Batch(nets, 1.0, 1.0, precision=N.PREC_MIXED, mixed_switch=args.switch, **shape)
+    progress("batched mixed: created")
+    reps, wall, kernel_ms = run_batch(mixed, args.steps, args.warmup)
+    n_conv = int(sum(r["status"] == N.CONVERGED and r["fmax"] < args.fcut for r in reps))
     value = n_conv * args.steps / wall
     t0d = min(r["t_start"] for r in reps)
     lat = np.array([r["t_done"] - t0d for r in reps])
     steps = np.array([r["steps"] for r in reps], dtype=np.float64)
-
-    # roofline of the relax kernel (one launch per step)
     total_bytes = 0.0
     node_steps = 0.0
     for k in range(len(specs)):
-        n_, e_, t_ = batch.counts(k)
-        total_bytes += reps[k]["steps"] * alg_bytes(n_, e_, t_)
+        n_, e_, t_ = mixed.counts(k)
+        s64 = reps[k]["fp64_steps"]
+        s32 = reps[k]["steps"] - s64
+        total_bytes += s32 * alg_bytes32(n_, e_, t_) + s64 * alg_bytes(n_, e_, t_)
         node_steps += reps[k]["steps"] * n_
-    achieved = total_bytes / (kernel_ms / args.steps * 1e-3) / 1e9
+    achieved = total_bytes / (kernel_ms * 1e-3) / 1e9
+    e_mixed = np.array([r["es"] + r["eb"] for r in reps])
+    del mixed
+
+    # the same batch in FP64, and the mixed form's energy error against it
+    fp64 = None
+    if not args.no_fp64:
+        b64 = N.Batch(nets, 1.0, 1.0, precision=N.PREC_FP64, **shape)
+        progress("batched fp64: created")
+        r64, wall64, kms64 = run_batch(b64, 1, 0)
+        e64 = np.array([r["es"] + r["eb"] for r in r64])
+        rel = np.abs(e_mixed - e64) / np.abs(e64)
+        fp64 = {"networks_per_s": sum(r["status"] == N.CONVERGED for r in r64) / wall64,
+                "kernel_ms": kms64,
+                "mixed_energy_rel_diff": {"median": float(np.median(rel)),
+                                          "p90": float(np.percentile(rel, 90)),
+                                          "max": float(rel.max()),
+                                          "n_over_1e-10": int((rel > 1e-10).sum()),
+                                          "n": len(rel)}}
+        del b64
 
     # plain form: one network per launch, FP64, on a sample
     plain = None
@@ -138,13 +182,20 @@ def main():
             r1 = b1.relax(prm)[0]
             t_plain += time.perf_counter() - ts
             ok += r1["status"] == N.CONVERGED
+            progress(f"plain: network {k} (L={specs[k]['L']}) {r1['steps']} steps")
             del b1
         plain = dict(networks_per_s=ok / t_plain, sample=len(idx),
-                     sample_nodes=int(sum(specs[k]["L"] ** 2 for k in idx)))
+                     sample_nodes=int(sum(specs[k]["L"] ** 2 for k in idx)),
+                     form="one network per launch, FP64, widest team (tiles_per_member=1)")
+
+    served = None
+    if args.served > 0:
+        served = served_leg(N, args)
 
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, reps, args)
+        progress("cpu baseline done")
 
     out = {
         "metric": "relaxed networks/s at |F|max<1e-9: batched+mixed vs plain FIRE; p99 latency",
@@ -157,28 +208,116 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f32+f64",
         "data": "synthetic",
         "config": {"workload": f"batched: {len(specs)} Lees-Edwards triangular networks, L even "
                                f"U[{args.lmin},{args.lmax}], p U[0.55,0.75], per-fiber kappa "
                                f"log-U[1e-5,1e-2], gamma U[0,0.4] affine start, FIRE to "
-                               f"|F|max<{args.fcut:g}",
+                               f"|F|max<{args.fcut:g}, mixed precision (FP32 stage on FP64 "
+                               f"anchors, FP64 polish)",
                    "networks": len(specs), "nodes": int(sum(s["L"] ** 2 for s in specs)),
-                   "precision": "fp64", "fire": FIRE, "parallelism": "1 GPU, persistent batch"},
+                   "precision": "mixed", "fire": FIRE, "parallelism": "1 GPU, persistent batch"},
         "converged": n_conv,
         "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                       "max": float(lat.max())},
         "fire_steps": {"mean": float(steps.mean()), "max": int(steps.max()),
+                       "fp64_mean": float(np.mean([r["fp64_steps"] for r in reps])),
                        "node_steps": node_steps},
+        "batched_fp64": fp64,
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["networks_per_s"]) if plain else None,
+        "served": served,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "relax_kernel", "kernel_ms": kernel_ms / args.steps,
+                     "kernel": "relax_kernel<256,1>", "kernel_ms": kernel_ms,
                      "alg_bytes_per_launch": total_bytes},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def served_leg(N, args):
+    """Config-5 style stream through the serving loop: requests with L
+    log-uniform in [16, served_lmax], p U[0.55,0.75], per-fiber kappa, 1-20
+    shear increments of U[0.01,0.05], mixed precision.  First all requests at
+    once (saturated throughput X), then Poisson arrivals at 0.7 X with
+    latency = arrival -> last increment done."""
+    rng = np.random.RandomState(args.seed + 5)
+    n = args.served
+    prm = N.relax_params(fcut=args.fcut, max_steps=args.max_steps, criterion=N.CRIT_MAX, **FIRE)
+    reqs = []
+    for i in range(n):
+        L = int(round(np.exp(rng.uniform(np.log(16), np.log(args.served_lmax))) / 2)) * 2
+        L = max(L, 16)
+        p = rng.uniform(0.55, 0.75)
+        ninc = int(rng.randint(1, 21))
+        d = rng.uniform(0.01, 0.05, size=ninc)
+        net = N.Network.lattice(L, p, 9000 + i, True).triples().fiber_kappa(1e-5, 1e-2, 7000 + i)
+        yr = net.info().height
+        incs = np.stack([d * yr, np.zeros(ninc)], axis=1)
+        reqs.append((net, incs))
+    progress(f"served: built {n} requests")
+
+    def run(arrivals):
+        srv = N.Server(precision=N.PREC_MIXED, slice_steps=1000)
+        t0 = srv.clock()
+        i = done = 0
+        res = {}
+        ids = {}
+        last = time.perf_counter()
+        while done < n:
+            if time.perf_counter() - last > 5.0:
+                last = time.perf_counter()
+                info = srv.stats()
+                progress(f"served: submitted {i}/{n} done {done} active {info['active']} "
+                         f"waiting {info['waiting']} launches {info['slices']}")
+            now = srv.clock() - t0
+            while i < n and arrivals[i] <= now:
+                ids[srv.submit(reqs[i][0], 1.0, 1.0, prm, reqs[i][1])] = i
+                i += 1
+            act, wait = srv.step()
+            for r in srv.results():
+                res[ids[r["job"]]] = r
+                done += 1
+            if act == 0 and wait == 0 and i < n:
+                time.sleep(max(0.0, arrivals[i] - (srv.clock() - t0)))
+        wall = max(r["t_done"] for r in res.values()) - t0
+        lat = np.array([res[k]["t_done"] - res[k]["t_submit"] for k in range(n)])
+        ok = sum(res[k]["status"] == N.CONVERGED for k in range(n))
+        info = srv.stats()
+        return res, wall, lat, ok, info
+
+    _, wall_sat, _, ok_sat, _ = run(np.zeros(n))
+    x_sat = ok_sat / wall_sat
+    progress(f"served: saturated {x_sat:.2f} jobs/s")
+    lam = 0.7 * x_sat
+    arr = np.cumsum(rng.exponential(1.0 / lam, size=n))
+    res, wall, lat, ok, info = run(arr)
+    progress(f"served: poisson at {lam:.2f}/s done")
+    # mixed vs FP64, per increment, on a sample of jobs (FP64 ramp on the GPU)
+    errs = []
+    for k in range(0, n, max(1, n // 12)):
+        b = N.Batch([reqs[k][0]], 1.0, 1.0, precision=N.PREC_FP64)
+        ramp = b.ramp(reqs[k][1], prm)[0]
+        e_m = np.array([x["es"] + x["eb"] for x in res[k]["reports"]])
+        e_f = np.array([x["es"] + x["eb"] for x in ramp])
+        errs.extend(list(np.abs(e_m - e_f) / np.abs(e_f)))
+        del b
+    errs = np.array(errs)
+    return {"requests": n, "converged": int(ok),
+            "workload": f"L log-U[16,{args.served_lmax}], p U[0.55,0.75], kappa log-U[1e-5,1e-2] "
+                        f"per fiber, 1-20 shear increments U[0.01,0.05], mixed precision, "
+                        f"1000-step slices",
+            "increments": int(sum(len(r[1]) for r in reqs)),
+            "saturated_networks_per_s": x_sat,
+            "poisson_rate_per_s": lam,
+            "networks_per_s": ok / wall,
+            "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "max": float(lat.max())},
+            "launches": info["slices"], "device_busy_frac": info["busy_ms"] * 1e-3 / wall,
+            "mixed_vs_fp64_energy_rel": {"median": float(np.median(errs)), "max": float(errs.max()),
+                                         "n_over_1e-10": int((errs > 1e-10).sum()),
+                                         "n": int(len(errs))}}
 
 
 def calibrate_copy(mib):
@@ -205,7 +344,7 @@ def cpu_baseline(specs, reps, args):
     cores = min(16, os.cpu_count() or 1)
     rng = np.random.RandomState(args.seed + 2)
     idx = rng.choice(len(specs), size=min(cores, len(specs)), replace=False)
-    per_net_steps = 400
+    per_net_steps = 6000
 
     def run(k):
         s = specs[k]

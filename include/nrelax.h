@@ -98,7 +98,7 @@ typedef struct {
                              T / tiles_per_member workgroups, 0 = tuned */
     double mixed_switch;  /* NR_PREC_MIXED: the FP32 stage hands over to FP64
                              when max |F| < mixed_switch x its initial value,
-                             0 = default (1e-4) */
+                             0 = default (3e-2) */
 } nr_batch_opts;
 
 /* ---- errors ------------------------------------------------------------ */

@@ -63,7 +63,7 @@ struct LaunchDev {
     nr::TeamLayout L;
     DevBuf<nr::BlockRole> roles;
     DevBuf<int> class_jobs, class_off, queues, mailbox;
-    DevBuf<double> partials;
+    DevBuf<unsigned long long> partials;
     DevBuf<unsigned long long> bar;
 };
 
@@ -82,7 +82,7 @@ struct nr_batch {
     std::vector<double> ks, kb;
     DevBuf<double2> x[2], v[2], f[2], x_init;
     DevBuf<float2> u32[2], v32[2], f32[2];  // mixed precision FP32 stage
-    double switch_ratio = 1e-4;  // FP32 stage stops at this fraction of the initial |F|max
+    double switch_ratio = 3e-2;  // FP32 stage stops at this fraction of the initial |F|max
     int stall_steps = 2000;      // ... or after this many steps without a new minimum
     DevBuf<unsigned char> mask;
     DevBuf<uint4> blob;
@@ -221,7 +221,9 @@ static int choose_shapes(nr_batch *b, const nr_batch_opts *opts, int64_t *lds_bu
     b->block = block;
     b->npt = tile > block ? 2 : 1;
     b->plan.tile_nodes = tile;
-    *lds_budget = block == 256 ? 80 * 1024 : 156 * 1024;
+    // two workgroups per CU at 256 threads (one at 512): the tile area gets
+    // what the LDS cache and the static scratch leave
+    *lds_budget = nr::lds_per_workgroup(block) - nr::kLdsReserve;
     return NR_OK;
 }
 
@@ -250,7 +252,7 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     rc = nr::build_plan(b->nets, b->ks, b->kb, b->plan.tile_nodes, lds_budget, b->plan, err);
     if (rc) return fail(rc, err);
     const nr::Plan &P = b->plan;
-    b->lds = (size_t)P.lds_bytes();
+    b->lds = (size_t)(nr::kCacheBytes + P.lds_bytes());
     if (b->lds > 160 * 1024) return fail(NR_E_TOPO, "tile working set exceeds LDS");
 
     // teams: each network gets a power-of-two team of workgroups (a function
@@ -281,7 +283,7 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
             HIP_TRY(ld->class_off.upload(L.class_off));
             HIP_TRY(ld->queues.alloc((size_t)32 * (L.class_off.size() - 1)));
             HIP_TRY(ld->mailbox.alloc((size_t)32 * std::max(1, L.n_teams)));
-            HIP_TRY(ld->partials.alloc((size_t)4 * std::max(1, L.n_parts)));
+            HIP_TRY(ld->partials.alloc((size_t)nr::kPartWords * std::max(1, L.n_parts)));
             HIP_TRY(ld->bar.alloc((size_t)nr::kBarStride * std::max(1, L.n_teams)));
             HIP_TRY(hipMemset(ld->bar.p, 0, sizeof(unsigned long long) * nr::kBarStride *
                                                  std::max(1, L.n_teams)));
@@ -406,6 +408,9 @@ static int run_kernel(nr_batch *b, int mode)
     HIP_TRY(hipMemsetAsync(b->run.p, 0, sizeof(nr::NetRun) * b->run.n, b->stream));
     for (auto &ld : b->launches) {
         HIP_TRY(hipMemsetAsync(ld->queues.p, 0, sizeof(int) * ld->queues.n, b->stream));
+        // partial-sum records carry per-launch step tags starting at 1
+        HIP_TRY(hipMemsetAsync(ld->partials.p, 0, sizeof(unsigned long long) * ld->partials.n,
+                               b->stream));
         a.roles = ld->roles.p;
         a.class_jobs = ld->class_jobs.p;
         a.class_off = ld->class_off.p;

@@ -79,7 +79,7 @@ struct KArgs {
     int *queues;              // per class job counter, stride 32
     int max_blob, max_local, max_fv;
     int mode;
-    double *partials;         // 4 doubles per team member
+    unsigned long long *partials;  // per team member kPartWords tagged words (zeroed per launch)
     unsigned long long *bar;  // per team kBarStride words: see team_arrive
     int *mailbox;             // per team, stride 32
     unsigned *err;            // nonzero after a bounded wait expired
@@ -89,6 +89,24 @@ struct KArgs {
     NetOut *inc_out;          // one report per increment, at NetDev::out_base
     int slice_steps;          // FIRE steps per network per launch, 0 = no limit
 };
+
+// LDS cache, per workgroup, of the tile descriptors and halo node ids of the
+// first kTileCache tiles a team member relaxes in its current network (halo
+// lists up to kCacheHalo ids), placed before the tile area: with the halo ids
+// at hand a tile pass issues its blob, own-node and halo-node loads at once.
+constexpr int kTileCache = 6;
+constexpr int kCacheHalo = 192;
+constexpr int kCacheBytes = kTileCache * (int)sizeof(TileDev) + kTileCache * kCacheHalo * 4;
+static_assert(kCacheBytes % 16 == 0, "tile area alignment");
+// LDS of a workgroup outside the tile area: the cache and static scratch.
+constexpr int kLdsReserve = kCacheBytes + 1024;
+// LDS of one workgroup: two 256-thread workgroups share a CU's 160 KB, a
+// 512-thread workgroup has it alone.
+inline int lds_per_workgroup(int block) { return block == 256 ? 80 * 1024 : 156 * 1024; }
+
+// Partial-sum record of a team member: two step parities x 8 words, each
+// word = step tag (high 32 bits) | one half of a double (low 32 bits).
+constexpr int kPartWords = 16;
 
 // Team barrier words per team: [0] root arrivals, [16] generation,
 // [32 + 16 g] arrivals of group g (groups of kBarGroup members).

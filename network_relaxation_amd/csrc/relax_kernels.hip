@@ -142,7 +142,59 @@ __device__ inline void drain_memory()
 struct Team {
     int id, member, g, part;
     unsigned long long gen;
+    unsigned rtag;  // reductions so far in this launch (tag of the partial records)
 };
+
+// A member publishes its partial sums as 8 tagged 64-bit words (high half the
+// step tag, low half 32 bits of a double), so a reader needs no barrier: it
+// accepts a record when all 8 words carry the tag of the step it waits for.
+__device__ inline void put_record(unsigned long long *w, unsigned tag, const Red4 &t)
+{
+    const unsigned long long hi = (unsigned long long)tag << 32;
+    const double v[4] = {t.a, t.b, t.c, t.m};
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v[i]);
+        __hip_atomic_store(w + 2 * i, hi | (b >> 32), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(w + 2 * i + 1, hi | (b & 0xFFFFFFFFull), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+__device__ inline double join_halves(unsigned long long hi, unsigned long long lo)
+{
+    return __longlong_as_double((long long)(((hi & 0xFFFFFFFFull) << 32) | (lo & 0xFFFFFFFFull)));
+}
+
+// Waits (bounded) until the record at w carries `tag`; false when the wait
+// bound expired or another workgroup reported an expired wait.
+__device__ bool poll_record(const KArgs &A, const unsigned long long *w, unsigned tag, Red4 &v)
+{
+    const long long t0 = wall_clock64();
+    for (;;) {
+        unsigned long long x[8];
+#pragma unroll
+        for (int i = 0; i < 8; i++)
+            x[i] = __hip_atomic_load(const_cast<unsigned long long *>(w + i), __ATOMIC_RELAXED,
+                                     __HIP_MEMORY_SCOPE_AGENT);
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < 8; i++) ok = ok && (unsigned)(x[i] >> 32) == tag;
+        if (ok) {
+            v.a = join_halves(x[0], x[1]);
+            v.b = join_halves(x[2], x[3]);
+            v.c = join_halves(x[4], x[5]);
+            v.m = join_halves(x[6], x[7]);
+            return true;
+        }
+        if (wall_clock64() - t0 > A.timeout_ticks ||
+            __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
+            __hip_atomic_store(A.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            return false;
+        }
+        __builtin_amdgcn_s_sleep(1);
+    }
+}
 
 // Bounded two-level team barrier, executed by thread 0 of each member after
 // its workgroup drained its stores.  Arrival counters only grow: the member
@@ -203,6 +255,8 @@ template <int B>
 __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_red, int *s_flag)
 {
     constexpr int W = B / 64;
+    const bool team = T.g > 1;
+    const unsigned tag = team ? ++T.rtag : 0u;
     r.a = wave_sum(r.a);
     r.b = wave_sum(r.b);
     r.c = wave_sum(r.c);
@@ -214,6 +268,9 @@ __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_re
         s_red[4 * wave + 2] = r.c;
         s_red[4 * wave + 3] = r.m;
     }
+    // a member's record is published after every wave's state stores have
+    // completed, so a member that reads it also sees that member's state
+    if (team) drain_memory();
     __syncthreads();
     if (threadIdx.x == 0) {
         Red4 t = {0.0, 0.0, 0.0, 0.0};
@@ -227,28 +284,21 @@ __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_re
         s_red[4 * W + 1] = t.b;
         s_red[4 * W + 2] = t.c;
         s_red[4 * W + 3] = t.m;
-        if (T.g > 1) {
-            double *slot = A.partials + ((size_t)T.part + T.member) * 4;
-            st_shared_f64(slot + 0, t.a);
-            st_shared_f64(slot + 1, t.b);
-            st_shared_f64(slot + 2, t.c);
-            st_shared_f64(slot + 3, t.m);
+        if (team) {
+            *s_flag = 1;
+            put_record(A.partials + (size_t)(T.part + T.member) * kPartWords + (tag & 1) * 8, tag, t);
         }
     }
-    if (T.g > 1) {
-        if (!team_sync(A, T, s_flag)) return false;
+    if (team) {
+        __syncthreads();
         // members' partials, one per thread, reduced by the same fixed tree
-        const double *base = A.partials + (size_t)T.part * 4;
         Red4 t = {0.0, 0.0, 0.0, 0.0};
         for (int m0 = 0; m0 < T.g; m0 += B) {
             const int m = m0 + (int)threadIdx.x;
             Red4 v = {0.0, 0.0, 0.0, 0.0};
-            if (m < T.g) {
-                v.a = ld_shared_f64(base + 4 * m + 0);
-                v.b = ld_shared_f64(base + 4 * m + 1);
-                v.c = ld_shared_f64(base + 4 * m + 2);
-                v.m = ld_shared_f64(base + 4 * m + 3);
-            }
+            if (m < T.g &&
+                !poll_record(A, A.partials + (size_t)(T.part + m) * kPartWords + (tag & 1) * 8, tag, v))
+                *s_flag = 0;
             v.a = wave_sum(v.a);
             v.b = wave_sum(v.b);
             v.c = wave_sum(v.c);
@@ -281,8 +331,9 @@ __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_re
     out.b = s_red[4 * W + 1];
     out.c = s_red[4 * W + 2];
     out.m = s_red[4 * W + 3];
+    const bool ok = !team || *s_flag != 0;
     __syncthreads();
-    return true;
+    return ok;
 }
 
 // LDS regions: the tile blob, then positions and force vectors.
@@ -290,7 +341,41 @@ struct Lds {
     unsigned char *blob;
     double2 *pos;
     double2 *fv;
+    TileDev *ctd;  // cached tile descriptors of this member
+    int *chalo;    // their halo ids, kCacheHalo per tile
 };
+
+// Descriptor of the kt-th tile of this member: from the LDS cache (every
+// field moved to scalar registers) or, past the cache, from global memory.
+// Built field by field so that it stays in registers.
+__device__ inline TileDev tile_desc(const KArgs &A, const Lds &L0, int idx, int kt)
+{
+    TileDev u;
+    if (kt < kTileCache) {
+        const TileDev &t = L0.ctd[kt];
+        u.net = __builtin_amdgcn_readfirstlane(t.net);
+        u.node0 = __builtin_amdgcn_readfirstlane(t.node0);
+        u.n_own = __builtin_amdgcn_readfirstlane(t.n_own);
+        u.n_halo = __builtin_amdgcn_readfirstlane(t.n_halo);
+        u.nb = __builtin_amdgcn_readfirstlane(t.nb);
+        u.nt = __builtin_amdgcn_readfirstlane(t.nt);
+        u.n_inc = __builtin_amdgcn_readfirstlane(t.n_inc);
+        u.blob16 = __builtin_amdgcn_readfirstlane(t.blob16);
+        u.blob_bytes = __builtin_amdgcn_readfirstlane(t.blob_bytes);
+    } else {
+        const TileDev *t = A.tiles + idx;
+        u.net = t->net;
+        u.node0 = t->node0;
+        u.n_own = t->n_own;
+        u.n_halo = t->n_halo;
+        u.nb = t->nb;
+        u.nt = t->nt;
+        u.n_inc = t->n_inc;
+        u.blob16 = t->blob16;
+        u.blob_bytes = t->blob_bytes;
+    }
+    return u;
+}
 
 // Views into a tile blob held in LDS.
 struct Blob {
@@ -728,12 +813,18 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     const double2 *__restrict__ F = A.f[p];
     const double2 zero = make_double2(0.0, 0.0);
     const bool sh = T.g > 1;
-    for (int ti = T.member; ti < C.n_tiles; ti += T.g) {
-        const TileDev td = A.tiles[C.tile_base + ti];
+    for (int ti = T.member, kt = 0; ti < C.n_tiles; ti += T.g, kt++) {
+        const bool cached = kt < kTileCache;
+        const TileDev td = tile_desc(A, L0, C.tile_base + ti, kt);
         // positions and force vectors follow this tile's blob
         Lds L = L0;
         L.pos = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
         L.fv = L.pos + td.n_own + td.n_halo;
+        // halo ids already in LDS (resident blob or the member's cache): the
+        // blob, own-node and halo-node loads then go out together
+        const int *hid = resident ? reinterpret_cast<const int *>(L0.blob)
+                         : (cached && td.n_halo <= kCacheHalo) ? L0.chalo + kt * kCacheHalo
+                                                               : nullptr;
         // The blob's first kBlobRound x 16 B per thread and the own nodes'
         // state are loaded together; the LDS writes then wait only for the
         // loads they use.
@@ -775,6 +866,21 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 if (KIND != kPassEnergy) mk[j] = A.mask[gid];
             }
         }
+        // first halo node of this thread (the rest, if any, below)
+        const bool hfirst = hid != nullptr && tid < td.n_halo;
+        double2 hx = zero, hv = zero, hf = zero;
+        if (hfirst) {
+            const int gid = hid[tid];
+            if (KIND == kPassFuse) {
+                hx = fuse_position(sh, A, S, X, gid);
+            } else {
+                hx = ld_state(sh, X + gid);
+                if (KIND == kPassStep) {
+                    hv = ld_state(sh, V + gid);
+                    hf = ld_state(sh, F + gid);
+                }
+            }
+        }
         if (!resident) {
             if (tid < n16) bdst[tid] = r0;
             if (tid + B < n16) bdst[tid + B] = r1;
@@ -812,10 +918,21 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 L.pos[k] = xn;
             }
         }
-        if (!resident) __syncthreads();  // halo ids come from the blob
+        // halo nodes: advanced here as their owners advance them
+        if (hfirst) {
+            double2 dummy, xn;
+            if (KIND == kPassEnergy || KIND == kPassFuse)
+                xn = hx;
+            else if (KIND == kPassInit)
+                xn = advance<kPassInit>(hx, zero, zero, S, C, dummy);
+            else
+                xn = advance<kPassStep>(hx, hv, hf, S, C, dummy);
+            L.pos[td.n_own + tid] = xn;
+        }
+        if (hid == nullptr) __syncthreads();  // halo ids come from the blob
         const Blob bb = blob_view(L.blob, td);
-        for (int k = td.n_own + tid; k < td.n_own + td.n_halo; k += B) {
-            const int gid = bb.halo[k - td.n_own];
+        for (int k = td.n_own + (hid != nullptr ? B : 0) + tid; k < td.n_own + td.n_halo; k += B) {
+            const int gid = hid != nullptr ? hid[k - td.n_own] : bb.halo[k - td.n_own];
             double2 dummy, xn;
             if (KIND == kPassEnergy)
                 xn = ld_state(sh, X + gid);
@@ -934,10 +1051,21 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
 
     // a member owning exactly one tile keeps that tile's blob in LDS
     const bool resident = T.member < C.n_tiles && C.n_tiles <= T.member + T.g;
-    if (resident) {
-        stage_blob<B>(A, A.tiles[C.tile_base + T.member], L.blob);
-        __syncthreads();
+    if (resident) stage_blob<B>(A, A.tiles[C.tile_base + T.member], L.blob);
+    // descriptors and halo ids of the member's first tiles (halo ids lead the blob)
+    for (int k = 0, ti = T.member; k < kTileCache && ti < C.n_tiles; k++, ti += T.g) {
+        const TileDev *tg = A.tiles + C.tile_base + ti;
+        constexpr int kWords = (int)(sizeof(TileDev) / sizeof(int));
+        if (threadIdx.x < kWords)
+            reinterpret_cast<int *>(L.ctd + k)[threadIdx.x] =
+                reinterpret_cast<const int *>(tg)[threadIdx.x];
+        const int n_halo = tg->n_halo;
+        if (n_halo <= kCacheHalo) {
+            const int *src = reinterpret_cast<const int *>(A.blob + tg->blob16);
+            for (int i = threadIdx.x; i < n_halo; i += B) L.chalo[k * kCacheHalo + i] = src[i];
+        }
     }
+    __syncthreads();
 
     Fire fire;
     fire.nd = nd;
@@ -1184,7 +1312,9 @@ __global__ __launch_bounds__(B) void relax_kernel(KArgs A)
     __shared__ int s_flag;
     __shared__ int s_job;
     Lds L;
-    L.blob = smem;
+    L.ctd = reinterpret_cast<TileDev *>(smem);
+    L.chalo = reinterpret_cast<int *>(smem + kTileCache * sizeof(TileDev));
+    L.blob = smem + kCacheBytes;
     L.pos = nullptr;  // placed per tile, after its blob
     L.fv = nullptr;
 
@@ -1196,6 +1326,7 @@ __global__ __launch_bounds__(B) void relax_kernel(KArgs A)
     T.g = role.g;
     T.part = role.part;
     T.gen = 0;
+    T.rtag = 0;
     if (T.g > 1 && threadIdx.x == 0)
         T.gen = __hip_atomic_load(A.bar + (size_t)T.id * kBarStride + 16, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT);

@@ -119,7 +119,7 @@ struct Job {
 struct Tables {
     DevBuf<nr::BlockRole> roles;
     DevBuf<int> class_jobs, class_off, queues, mailbox;
-    DevBuf<double> partials;
+    DevBuf<unsigned long long> partials;
     DevBuf<unsigned long long> bar;
     std::vector<nr::BlockRole> h_roles;  // host copies, alive until the launch ends
     std::vector<int> h_jobs, h_off;
@@ -134,7 +134,7 @@ constexpr int kMaxClasses = 64;
 struct nr_server {
     int device = 0, precision = NR_PREC_FP64, block = 256, npt = 1, tile_nodes = 256;
     int team_cap = 256, tiles_per_member = 4, slice_steps = 2000, stall_steps = 2000;
-    double switch_ratio = 1e-4;
+    double switch_ratio = 3e-2;
     int64_t lds_budget = 80 * 1024;
     size_t lds = 0;
     int resident = 0;
@@ -195,7 +195,7 @@ struct nr_server {
             std::vector<const nr::Network *> nets{&j->net};
             std::vector<double> ks{j->ks}, kb{j->kb};
             j->plan_rc = nr::build_plan(nets, ks, kb, tile_nodes, lds_budget, j->plan, j->plan_err);
-            if (!j->plan_rc && j->plan.lds_bytes() > (int64_t)lds) {
+            if (!j->plan_rc && j->plan.lds_bytes() > lds_budget) {
                 j->plan_rc = NR_E_TOPO;
                 j->plan_err = "tile working set exceeds the server's LDS size";
             }
@@ -421,7 +421,7 @@ int launch_slice(nr_server *s)
         HIP_TRY(t->class_off.alloc(kMaxClasses + 1));
         HIP_TRY(t->queues.alloc(32 * kMaxClasses));
         HIP_TRY(t->mailbox.alloc((size_t)32 * R));
-        HIP_TRY(t->partials.alloc((size_t)4 * R));
+        HIP_TRY(t->partials.alloc((size_t)nr::kPartWords * R));
         HIP_TRY(t->bar.alloc((size_t)nr::kBarStride * R));
         s->tables.push_back(std::move(t));
     }
@@ -470,6 +470,9 @@ int launch_slice(nr_server *s)
         HIP_TRY(hipMemcpyAsync(t.class_off.p, t.h_off.data(), sizeof(int) * t.h_off.size(),
                                hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemsetAsync(t.queues.p, 0, sizeof(int) * 32 * ncls, s->stream));
+        // partial-sum records carry per-launch step tags starting at 1
+        HIP_TRY(hipMemsetAsync(t.partials.p, 0, sizeof(unsigned long long) * t.partials.n,
+                               s->stream));
         // team compositions change between slices: restart the barrier counters
         HIP_TRY(hipMemsetAsync(t.bar.p, 0,
                                sizeof(unsigned long long) * nr::kBarStride * std::max(1, t.n_teams),
@@ -572,8 +575,8 @@ int nr_server_create(const nr_batch_opts *shape, const nr_server_opts *opts, nr_
     if (s->precision == NR_PREC_MIXED && s->tile_nodes > s->block)
         return fail(NR_E_ARG, "mixed precision needs tile_nodes <= block");
     s->npt = s->tile_nodes > s->block ? 2 : 1;
-    s->lds_budget = s->block == 256 ? 80 * 1024 : 156 * 1024;
-    s->lds = (size_t)s->lds_budget;
+    s->lds_budget = nr::lds_per_workgroup(s->block) - nr::kLdsReserve;
+    s->lds = (size_t)(nr::kCacheBytes + s->lds_budget);
     s->tiles_per_member = shape && shape->tiles_per_member > 0 ? shape->tiles_per_member : 4;
     const int64_t hbm = opts && opts->hbm_bytes > 0 ? opts->hbm_bytes : (int64_t)8 << 30;
     s->slot_cap = opts && opts->max_jobs > 0 ? opts->max_jobs : 4096;
