@@ -73,6 +73,7 @@ struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
     int npt = 1, block = 256, team_cap = 256, tiles_per_member = 4;
+    int method = NR_METHOD_FIRE;  // of the last relax call (one method per call)
     size_t lds = 0;
     std::vector<std::unique_ptr<LaunchDev>> launches;
     std::vector<int> team_of_net;
@@ -418,7 +419,8 @@ static int run_kernel(nr_batch *b, int mode)
         a.partials = ld->partials.p;
         a.bar = ld->bar.p;
         a.mailbox = ld->mailbox.p;
-        int e = nr::launch_relax(a, b->block, b->npt, ld->L.grid, b->lds, ld->L.cooperative,
+        const int method = mode == nr::kModeEval ? NR_METHOD_FIRE : b->method;
+        int e = nr::launch_relax(a, b->block, b->npt, method, ld->L.grid, b->lds, ld->L.cooperative,
                                  b->stream);
         if (e) return hip_fail((hipError_t)e, "relax kernel launch");
     }
@@ -440,9 +442,14 @@ static int run_kernel(nr_batch *b, int mode)
 
 static int set_params(nr_batch *b, const nr_relax_params *params, int shared)
 {
+    for (size_t k = 0; k < b->hnet.size(); k++)
+        if ((shared ? params[0] : params[k]).method != params[0].method)
+            return fail(NR_E_ARG, "one relax call uses one method for every network");
+    b->method = params[0].method;
     for (size_t k = 0; k < b->hnet.size(); k++) {
         const nr_relax_params &p = shared ? params[0] : params[k];
-        if (p.method != NR_METHOD_FIRE) return fail(NR_E_ARG, "method not available");
+        if (p.method != NR_METHOD_FIRE && p.method != NR_METHOD_CG)
+            return fail(NR_E_ARG, "unknown relax method");
         if (!(p.mass > 0) || p.max_steps < 0 || p.fcut < 0)
             return fail(NR_E_ARG, "invalid relax parameters");
         nr::NetDev &d = b->hnet[k];

@@ -27,16 +27,19 @@ struct NetDev {
 // Resumable relax state of one network (all zero = start a fresh relax).
 // A launch with a step budget stops a relax mid-way and saves here what the
 // next launch needs to continue it with the same arithmetic.
-enum RunPhase { kRunFresh = 0, kRun32 = 1, kRun64 = 2 };
+enum RunPhase { kRunFresh = 0, kRun32 = 1, kRun64 = 2, kRunCg = 3 };
 struct NetRun {
     int phase;
     int step, steps32;       // FIRE steps of the current increment, FP32 stage
     int since;               // FIRE steps since the power was last negative
     int since_best;          // FP32 stage: steps without a new minimum
-    int mix, s;              // deferred mix of the next pass; FP32 parity
+    int mix, s;              // deferred mix of the next pass; FP32 parity (CG:
+                             // parity of the direction buffer)
     int inc_next;            // increments of the list already started
-    double alpha, dt;        // FIRE mixing factor and time step
-    double alpha_old, cmix;  // deferred mix coefficients
+    double alpha, dt;        // FIRE mixing factor and time step (CG: next beta,
+                             // line-search step length)
+    double alpha_old, cmix;  // deferred mix coefficients (CG: |d|^2, F.d of
+                             // the last accepted trial)
     double best, fswitch;    // FP32 stage: least max |F|^2, hand-over threshold
     double ff, fmax2;        // |F|^2 and max node |F|^2 after the last pass
     long long t_start;       // device clock when the increment began
@@ -121,8 +124,9 @@ inline size_t kernel_lds(int max_blob, int max_local, int max_fv)
 
 // Host entry: launches the persistent relax kernel (grid workgroups of
 // `block` threads, npt own nodes per thread).  Returns a hipError_t value.
-int launch_relax(const KArgs &a, int block, int npt, int grid, size_t lds, bool cooperative,
-                 void *stream);
+// method: NR_METHOD_FIRE or NR_METHOD_CG (every network of the launch).
+int launch_relax(const KArgs &a, int block, int npt, int method, int grid, size_t lds,
+                 bool cooperative, void *stream);
 int max_resident_blocks(int block, int npt, size_t lds, int *per_cu, int *n_cu);
 
 }  // namespace nr

@@ -18,6 +18,8 @@
 // affine_displace :527-537.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "device.hpp"
 
 #pragma clang fp contract(off)
@@ -28,8 +30,16 @@ namespace {
 
 // kPassFuse starts the FP64 polish of a mixed-precision relax: x = FP64
 // anchor + FP32 displacement, then forces as in kPassInit.
-enum PassKind { kPassInit = 0, kPassStep = 1, kPassEnergy = 2, kPassFuse = 3 };
+// kPassCgTrial is one line-search trial of nonlinear CG: direction
+// d = f + beta d_old (first trial of an iteration; later trials reload it),
+// trial point x + a d from the iteration's base point, forces there, and the
+// reduction (F.d, -, |F|^2, max |F_node|^2).
+enum PassKind { kPassInit = 0, kPassStep = 1, kPassEnergy = 2, kPassFuse = 3, kPassCgTrial = 4 };
 enum MixKind { kMixNone = 0, kMixFire = 1, kMixZero = 2 };
+// CG line search: at most this many secant trials, accepted when
+// |phi'(a)| <= kCgEta |phi'(0)|.
+constexpr int kCgLineSearch = 12;
+constexpr double kCgEta = 1e-3;
 
 struct Red4 {
     double a, b, c, m;  // sums a, b, c and max m
@@ -41,6 +51,9 @@ struct StepScal {
     int mix;
     double pend_x, pend_y;   // affine step (init pass)
     int s;                   // parity of the FP32 state (mixed precision)
+    double cg_a, cg_beta;    // CG trial: step along d, Fletcher-Reeves beta
+    int cg_first;            // CG trial: form d (first trial) or reload it
+    int pd;                  // CG: parity of the direction buffer d_old
 };
 
 struct NetCtx {
@@ -451,6 +464,16 @@ __device__ inline double2 advance(const double2 x, const double2 v, const double
             xn.y = x.y + S.pend_y * (x.y - C.miny) / C.yrange;
         }
         vh = make_double2(0.0, 0.0);
+    } else if (KIND == kPassCgTrial) {
+        // d = f + beta d_old, then x + a d (trial points from the base point)
+        double2 d = v;
+        if (S.cg_first) {
+            d.x = fma(S.cg_beta, v.x, f.x);
+            d.y = fma(S.cg_beta, v.y, f.y);
+        }
+        xn.x = fma(S.cg_a, d.x, x.x);
+        xn.y = fma(S.cg_a, d.y, x.y);
+        vh = d;
     } else if (KIND == kPassStep) {
         double2 vm;
         if (S.mix == kMixFire) {
@@ -809,7 +832,10 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     const int tid = threadIdx.x;
     const int q = p ^ 1;
     const double2 *__restrict__ X = A.x[p];
-    const double2 *__restrict__ V = A.v[p];
+    // CG keeps its direction in the velocity buffers: d_old in v[pd], the
+    // iteration's d (written by its first trial) in v[pd ^ 1]
+    const double2 *__restrict__ V =
+        KIND == kPassCgTrial ? A.v[S.cg_first ? S.pd : S.pd ^ 1] : A.v[p];
     const double2 *__restrict__ F = A.f[p];
     const double2 zero = make_double2(0.0, 0.0);
     const bool sh = T.g > 1;
@@ -859,7 +885,7 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 } else {
                     xo[j] = ld_state(sh, X + gid);
                 }
-                if (KIND == kPassStep) {
+                if (KIND == kPassStep || KIND == kPassCgTrial) {
                     vo[j] = ld_state(sh, V + gid);
                     fo[j] = ld_state(sh, F + gid);
                 }
@@ -875,7 +901,7 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 hx = fuse_position(sh, A, S, X, gid);
             } else {
                 hx = ld_state(sh, X + gid);
-                if (KIND == kPassStep) {
+                if (KIND == kPassStep || KIND == kPassCgTrial) {
                     hv = ld_state(sh, V + gid);
                     hf = ld_state(sh, F + gid);
                 }
@@ -911,6 +937,10 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                     xn = xo[j];
                     vh[j] = vo[j];
                     st_state(sh, A.x[q] + gid, xn);
+                } else if (KIND == kPassCgTrial) {
+                    xn = advance<kPassCgTrial>(xo[j], vo[j], fo[j], S, C, vh[j]);
+                    st_state(sh, A.x[q] + gid, xn);
+                    if (S.cg_first) st_state(sh, A.v[S.pd ^ 1] + gid, vh[j]);
                 } else {
                     xn = advance<kPassStep>(xo[j], vo[j], fo[j], S, C, vh[j]);
                     st_state(sh, A.x[q] + gid, xn);
@@ -926,7 +956,7 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
             else if (KIND == kPassInit)
                 xn = advance<kPassInit>(hx, zero, zero, S, C, dummy);
             else
-                xn = advance<kPassStep>(hx, hv, hf, S, C, dummy);
+                xn = advance<KIND>(hx, hv, hf, S, C, dummy);
             L.pos[td.n_own + tid] = xn;
         }
         if (hid == nullptr) __syncthreads();  // halo ids come from the blob
@@ -941,8 +971,8 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
             else if (KIND == kPassFuse)
                 xn = fuse_position(sh, A, S, X, gid);
             else
-                xn = advance<kPassStep>(ld_state(sh, X + gid), ld_state(sh, V + gid),
-                                        ld_state(sh, F + gid), S, C, dummy);
+                xn = advance<KIND>(ld_state(sh, X + gid), ld_state(sh, V + gid),
+                                   ld_state(sh, F + gid), S, C, dummy);
             L.pos[k] = xn;
         }
         __syncthreads();
@@ -1001,17 +1031,23 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                     double2 fn = gather_force(bb, L.fv, k);
                     if (!(mk[j] & 1)) fn.x = 0.0;
                     if (!(mk[j] & 2)) fn.y = 0.0;
-                    double2 vn;
-                    vn.x = fma(S.dtm, fn.x, vh[j].x);  // :677 second half kick
-                    vn.y = fma(S.dtm, fn.y, vh[j].y);
-                    if (KIND == kPassInit) vn = zero;
-                    if (KIND == kPassFuse) vn = vh[j];  // the FP32 stage's velocity
                     st_state(sh, A.f[q] + gid, fn);
-                    st_state(sh, A.v[q] + gid, vn);
-                    acc.a = fma(vn.x, fn.x, acc.a);
-                    acc.a = fma(vn.y, fn.y, acc.a);
-                    acc.b = fma(vn.x, vn.x, acc.b);
-                    acc.b = fma(vn.y, vn.y, acc.b);
+                    if (KIND == kPassCgTrial) {
+                        // F . d along the trial direction
+                        acc.a = fma(fn.x, vh[j].x, acc.a);
+                        acc.a = fma(fn.y, vh[j].y, acc.a);
+                    } else {
+                        double2 vn;
+                        vn.x = fma(S.dtm, fn.x, vh[j].x);  // :677 second half kick
+                        vn.y = fma(S.dtm, fn.y, vh[j].y);
+                        if (KIND == kPassInit) vn = zero;
+                        if (KIND == kPassFuse) vn = vh[j];  // the FP32 stage's velocity
+                        st_state(sh, A.v[q] + gid, vn);
+                        acc.a = fma(vn.x, fn.x, acc.a);
+                        acc.a = fma(vn.y, fn.y, acc.a);
+                        acc.b = fma(vn.x, vn.x, acc.b);
+                        acc.b = fma(vn.y, vn.y, acc.b);
+                    }
                     const double f2 = fma(fn.x, fn.x, fn.y * fn.y);
                     acc.c += f2;
                     acc.m = fmax(acc.m, f2);
@@ -1023,7 +1059,10 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     return acc;
 }
 
-template <int B, int NPT>
+// CG selects the method compiled in: nonlinear CG, or FIRE (with the mixed
+// precision stage).  Each is its own kernel so that neither pays the other's
+// registers.
+template <int B, int NPT, bool CG>
 __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L, double *s_red,
                               int *s_flag)
 {
@@ -1113,7 +1152,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             step = 0;
             steps32 = 0;
             fire.start();
-            phase = A.mixed && max_steps > 0 ? kRun32 : kRun64;
+            phase = CG ? kRunCg : A.mixed && max_steps > 0 ? kRun32 : kRun64;
         } else {
             C.shift = shift;
             step = rs->step;
@@ -1129,7 +1168,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             S.s = rs->s;
         }
 
-        if (phase == kRun32) {
+        if (!CG && phase == kRun32) {
             // FP32 stage: anchor = positions after the strain step, in x[p ^ 1]
             const int pa = p ^ 1;
             double best, fswitch;
@@ -1204,7 +1243,100 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
         }
         shift = C.shift;
 
-        if (phase == kRun64 && !paused && status != NR_ABORTED) {
+        if (CG && phase == kRunCg && status != NR_ABORTED) {
+            // Nonlinear CG (Fletcher-Reeves, secant line search on
+            // phi'(a) = -F(x + a d).d): the algorithm of the FP64 oracle's CG.
+            // f.d and |d|^2 of the new direction d = f + beta d_old follow from
+            // the last accepted trial's reductions.
+            const double fcut = nd->fcut;
+            const int crit = nd->crit;
+            const double sqrt_dof = sqrt((double)(2 * C.n_nodes));
+            double beta_next, step_len, dd_old, fdt_acc;
+            int pd;
+            if (fresh) {
+                beta_next = 0.0;
+                step_len = 1e-2;
+                dd_old = 0.0;
+                fdt_acc = 0.0;
+                pd = p;  // zeroed by the init pass
+            } else {
+                beta_next = rs->alpha;
+                step_len = rs->dt;
+                dd_old = rs->alpha_old;
+                fdt_acc = rs->cmix;
+                pd = rs->s;
+            }
+            for (;;) {
+                const bool conv =
+                    crit == NR_CRIT_MAX ? sqrt(fmax2) < fcut : sqrt(ff) / sqrt_dof < fcut;
+                if (conv || !isfinite(ff) || step >= max_steps) break;
+                if (used >= budget) {
+                    paused = true;
+                    break;
+                }
+                step++;
+                used++;
+                double beta = beta_next;
+                double fd = fma(beta, fdt_acc, ff);
+                double dd = fma(beta * beta, dd_old, fma(2.0 * beta, fdt_acc, ff));
+                if (!(fd > 0)) {  // not a descent direction: restart along f
+                    beta = 0.0;
+                    fd = ff;
+                    dd = ff;
+                }
+                const double g0 = -fd;
+                double a_prev = 0.0, g_prev = g0, a = step_len / sqrt(dd);
+                Red4 tr = {0.0, 0.0, 0.0, 0.0};
+                S.cg_beta = beta;
+                S.pd = pd;
+                int ls = 0;
+                for (; ls < kCgLineSearch; ls++) {
+                    S.cg_a = a;
+                    S.cg_first = ls == 0;
+                    part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L, resident);
+                    if (!reduce4<B>(A, T, part, tr, s_red, s_flag)) {
+                        status = NR_ABORTED;
+                        break;
+                    }
+                    const double g = -tr.a;
+                    if (fabs(g) <= kCgEta * fabs(g0)) break;
+                    double an = a - g * (a - a_prev) / (g - g_prev);
+                    if (!(an > 0) || !isfinite(an)) an = 2.0 * a;
+                    if (an > 4.0 * a) an = 4.0 * a;
+                    a_prev = a;
+                    g_prev = g;
+                    a = an;
+                }
+                if (status == NR_ABORTED) break;
+                if (ls == kCgLineSearch) {  // evaluate the last step length
+                    S.cg_a = a;
+                    S.cg_first = 0;
+                    part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L, resident);
+                    if (!reduce4<B>(A, T, part, tr, s_red, s_flag)) {
+                        status = NR_ABORTED;
+                        break;
+                    }
+                }
+                // accept the trial point; its direction becomes d_old
+                step_len = a * sqrt(dd);
+                p ^= 1;
+                pd ^= 1;
+                beta_next = tr.c / ff;
+                fdt_acc = tr.a;
+                dd_old = dd;
+                ff = tr.c;
+                fmax2 = tr.m;
+            }
+            if (paused && lead) {
+                rs->alpha = beta_next;
+                rs->dt = step_len;
+                rs->alpha_old = dd_old;
+                rs->cmix = fdt_acc;
+                rs->s = pd;
+            }
+        }
+
+        if (!CG && phase == kRun64 && !paused && status != NR_ABORTED) {
             // run_fire_md :624-718
             const double fcut = nd->fcut;
             const int crit = nd->crit;
@@ -1278,13 +1410,15 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             rs->steps32 = steps32;
             rs->ff = ff;
             rs->fmax2 = fmax2;
-            rs->alpha = fire.alpha;
-            rs->dt = fire.dt;
-            rs->since = fire.since;
-            rs->alpha_old = S.alpha_old;
-            rs->cmix = S.cmix;
-            rs->mix = S.mix;
-            rs->s = S.s;
+            if (phase != kRunCg) {
+                rs->alpha = fire.alpha;
+                rs->dt = fire.dt;
+                rs->since = fire.since;
+                rs->alpha_old = S.alpha_old;
+                rs->cmix = S.cmix;
+                rs->mix = S.mix;
+                rs->s = S.s;
+            }
             NetOut o = {};
             o.rms = sqrt(ff) / sqrt((double)(2 * C.n_nodes));
             o.fmax = sqrt(fmax2);
@@ -1304,8 +1438,8 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     }
 }
 
-template <int B, int NPT>
-__global__ __launch_bounds__(B) void relax_kernel(KArgs A)
+template <int B, int NPT, bool CG>
+__device__ __forceinline__ void kernel_body(const KArgs &A)
 {
     extern __shared__ __align__(16) unsigned char smem[];
     __shared__ double s_red[4 * (B / 64 + 1)];
@@ -1350,26 +1484,45 @@ __global__ __launch_bounds__(B) void relax_kernel(KArgs A)
         const int job = s_job;
         __syncthreads();
         if (job >= n_jobs) return;
-        relax_network<B, NPT>(A, T, jobs[job], L, s_red, &s_flag);
+        relax_network<B, NPT, CG>(A, T, jobs[job], L, s_red, &s_flag);
         if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
     }
 }
 
-const void *kernel_for(int block, int npt)
+// FIRE (and mixed precision) and CG entry points; CG is held to two waves
+// per SIMD (256 VGPRs), the occupancy the FIRE kernel reaches by itself.
+template <int B, int NPT>
+__global__ __launch_bounds__(B) void relax_kernel(KArgs A)
 {
-    if (block == 256 && npt == 1) return (const void *)relax_kernel<256, 1>;
-    if (block == 256 && npt == 2) return (const void *)relax_kernel<256, 2>;
-    if (block == 512 && npt == 1) return (const void *)relax_kernel<512, 1>;
-    if (block == 512 && npt == 2) return (const void *)relax_kernel<512, 2>;
+    kernel_body<B, NPT, false>(A);
+}
+
+template <int B, int NPT>
+__global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(2))) void cg_kernel(KArgs A)
+{
+    kernel_body<B, NPT, true>(A);
+}
+
+const void *kernel_for(int block, int npt, int method)
+{
+    const bool cg = method == NR_METHOD_CG;
+    if (block == 256 && npt == 1)
+        return cg ? (const void *)cg_kernel<256, 1> : (const void *)relax_kernel<256, 1>;
+    if (block == 256 && npt == 2)
+        return cg ? (const void *)cg_kernel<256, 2> : (const void *)relax_kernel<256, 2>;
+    if (block == 512 && npt == 1)
+        return cg ? (const void *)cg_kernel<512, 1> : (const void *)relax_kernel<512, 1>;
+    if (block == 512 && npt == 2)
+        return cg ? (const void *)cg_kernel<512, 2> : (const void *)relax_kernel<512, 2>;
     return nullptr;
 }
 
 }  // namespace
 
-int launch_relax(const KArgs &a, int block, int npt, int grid, size_t lds, bool cooperative,
-                 void *stream)
+int launch_relax(const KArgs &a, int block, int npt, int method, int grid, size_t lds,
+                 bool cooperative, void *stream)
 {
-    const void *fn = kernel_for(block, npt);
+    const void *fn = kernel_for(block, npt, method);
     if (!fn) return (int)hipErrorInvalidValue;
     hipStream_t s = (hipStream_t)stream;
     void *args[] = {const_cast<KArgs *>(&a)};
@@ -1384,15 +1537,20 @@ int launch_relax(const KArgs &a, int block, int npt, int grid, size_t lds, bool 
 
 int max_resident_blocks(int block, int npt, size_t lds, int *per_cu, int *n_cu)
 {
-    const void *fn = kernel_for(block, npt);
-    if (!fn) return (int)hipErrorInvalidValue;
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-    if (e != hipSuccess) return (int)e;
-    int nb = 0;
-    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, fn, block, lds);
-    if (e != hipSuccess) return (int)e;
+    // the smaller residency of the FIRE and CG kernels
+    int nb = 1 << 30;
+    for (int method = NR_METHOD_FIRE; method <= NR_METHOD_CG; method++) {
+        const void *f = kernel_for(block, npt, method);
+        if (!f) return (int)hipErrorInvalidValue;
+        hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return (int)e;
+        int n = 0;
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, f, block, lds);
+        if (e != hipSuccess) return (int)e;
+        nb = std::min(nb, n);
+    }
     int dev = 0;
-    e = hipGetDevice(&dev);
+    hipError_t e = hipGetDevice(&dev);
     if (e != hipSuccess) return (int)e;
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, dev);

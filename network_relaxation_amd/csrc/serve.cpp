@@ -484,7 +484,8 @@ int launch_slice(nr_server *s)
         a.partials = t.partials.p;
         a.bar = t.bar.p;
         a.mailbox = t.mailbox.p;
-        int e = nr::launch_relax(a, s->block, s->npt, t.grid, s->lds, t.cooperative, s->stream);
+        int e = nr::launch_relax(a, s->block, s->npt, NR_METHOD_FIRE, t.grid, s->lds,
+                                 t.cooperative, s->stream);
         if (e) return hip_fail((hipError_t)e, "relax kernel launch");
     }
     HIP_TRY(hipEventRecord(s->ev1, s->stream));
@@ -646,7 +647,8 @@ int nr_server_submit(nr_server *s, const nr_network *net, double ks, double kb,
     if (!(ks >= 0) || !(kb >= 0)) return fail(NR_E_ARG, "negative stiffness");
     if (kb > 0 && !net->net.triples_built)
         return fail(NR_E_ARG, "bending stiffness > 0 but triples were not built");
-    if (params->method != NR_METHOD_FIRE) return fail(NR_E_ARG, "method not available");
+    if (params->method != NR_METHOD_FIRE)
+        return fail(NR_E_ARG, "the server relaxes by FIRE (CG runs through nr_batch_relax)");
     if (!(params->mass > 0) || params->max_steps < 0 || params->fcut < 0)
         return fail(NR_E_ARG, "invalid relax parameters");
     if (n_inc > 32 * 64) return fail(NR_E_ARG, "too many increments");
