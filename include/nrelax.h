@@ -161,6 +161,16 @@ int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared,
  * ramp there (its later reports are left untouched). */
 int nr_batch_ramp(nr_batch *b, int n_inc, const double *incs,
                   const nr_relax_params *params, int shared, nr_report *reports);
+/* One strain step of do_relaxation_run's loops (large_strain_relax.cu:
+ * 925-931, 967-973: affine_displace then run_fire_md) captured as a HIP
+ * graph: network k takes the affine step (xdisp[k], ydisp[k]) on the device,
+ * then every network relaxes with params.  nr_batch_graph_replay runs the
+ * whole step as one graph launch (the host's network records are uploaded
+ * first, so steps added with nr_batch_strain and positions set in between
+ * are honoured).  Capturing again replaces the graph. */
+int nr_batch_graph_capture(nr_batch *b, const double *xdisp, const double *ydisp,
+                           const nr_relax_params *params, int shared);
+int nr_batch_graph_replay(nr_batch *b, nr_report *reports);
 /* Restores every network to its input geometry (zero strain, no pending
  * step) without host traffic. */
 int nr_batch_reset(nr_batch *b);

@@ -117,6 +117,8 @@ def lib():
         L.nr_batch_destroy.restype = None
         L.nr_relaxation_batch.argtypes = [C.c_char_p, C.c_int, P(BatchOpts)]
         L.nr_batch_ramp.argtypes = [vp, C.c_int, dp, P(RelaxParams), C.c_int, P(Report)]
+        L.nr_batch_graph_capture.argtypes = [vp, dp, dp, P(RelaxParams), C.c_int]
+        L.nr_batch_graph_replay.argtypes = [vp, P(Report)]
         L.nr_server_create.argtypes = [P(BatchOpts), P(ServerOpts), P(vp)]
         L.nr_server_clock.argtypes = [vp]
         L.nr_server_clock.restype = C.c_double
@@ -274,6 +276,22 @@ class Batch:
             arr = (RelaxParams * self.count)(*params)
             _check(lib().nr_batch_ramp(self.h, n, _dp(incs), arr, 0, reps))
         return [[reps[k * n + i].as_dict() for i in range(n)] for k in range(self.count)]
+
+    def graph_capture(self, xdisp, ydisp, params):
+        """nr_batch_graph_capture: one strain step (per-network affine steps)
+        plus the relax, recorded as a HIP graph."""
+        xd = np.ascontiguousarray(np.broadcast_to(np.asarray(xdisp, np.float64), (self.count,)))
+        yd = np.ascontiguousarray(np.broadcast_to(np.asarray(ydisp, np.float64), (self.count,)))
+        if isinstance(params, RelaxParams):
+            _check(lib().nr_batch_graph_capture(self.h, _dp(xd), _dp(yd), C.byref(params), 1))
+        else:
+            arr = (RelaxParams * self.count)(*params)
+            _check(lib().nr_batch_graph_capture(self.h, _dp(xd), _dp(yd), arr, 0))
+
+    def graph_replay(self):
+        reps = (Report * self.count)()
+        _check(lib().nr_batch_graph_replay(self.h, reps))
+        return [r.as_dict() for r in reps]
 
     def reset(self):
         _check(lib().nr_batch_reset(self.h))

@@ -97,9 +97,28 @@ struct nr_batch {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
+    // captured strain step (nr_batch_graph_capture)
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    DevBuf<double2> gstep;             // per-network affine step of the graph
+    nr::NetOut *h_out = nullptr;       // pinned copies written by the graph
+    nr::NetDev *h_net = nullptr;
+    unsigned *h_err = nullptr;
+
+    void drop_graph()
+    {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (graph) (void)hipGraphDestroy(graph);
+        gexec = nullptr;
+        graph = nullptr;
+    }
 
     ~nr_batch()
     {
+        drop_graph();
+        if (h_out) (void)hipHostFree(h_out);
+        if (h_net) (void)hipHostFree(h_net);
+        if (h_err) (void)hipHostFree(h_err);
         if (ev0) (void)hipEventDestroy(ev0);
         if (ev1) (void)hipEventDestroy(ev1);
         if (stream) (void)hipStreamDestroy(stream);
@@ -369,12 +388,8 @@ int nr_batch_strain(nr_batch *b, int index, double xdisp, double ydisp)
     return NR_OK;
 }
 
-static int run_kernel(nr_batch *b, int mode)
+static void fill_kargs(const nr_batch *b, int mode, nr::KArgs &a)
 {
-    HIP_TRY(hipSetDevice(b->device));
-    HIP_TRY(hipMemcpyAsync(b->dnet.p, b->hnet.data(), sizeof(nr::NetDev) * b->hnet.size(),
-                           hipMemcpyHostToDevice, b->stream));
-    nr::KArgs a;
     std::memset(&a, 0, sizeof(a));
     for (int s = 0; s < 2; s++) {
         a.x[s] = b->x[s].p;
@@ -404,6 +419,15 @@ static int run_kernel(nr_batch *b, int mode)
     a.incs = b->incs.p;
     a.inc_out = b->inc_out.p;
     a.slice_steps = 0;
+}
+
+static int run_kernel(nr_batch *b, int mode)
+{
+    HIP_TRY(hipSetDevice(b->device));
+    HIP_TRY(hipMemcpyAsync(b->dnet.p, b->hnet.data(), sizeof(nr::NetDev) * b->hnet.size(),
+                           hipMemcpyHostToDevice, b->stream));
+    nr::KArgs a;
+    fill_kargs(b, mode, a);
     HIP_TRY(hipEventRecord(b->ev0, b->stream));
     // every call starts fresh relaxes
     HIP_TRY(hipMemsetAsync(b->run.p, 0, sizeof(nr::NetRun) * b->run.n, b->stream));
@@ -518,6 +542,103 @@ int nr_batch_ramp(nr_batch *b, int n_inc, const double *incs, const nr_relax_par
                           hipMemcpyDeviceToHost));
         for (size_t i = 0; i < o.size(); i++) nr::to_report(o[i], reports[i]);
     }
+    return NR_OK;
+}
+
+// Records one strain step of every network (pending step += its affine step,
+// fresh relax state, the relax launches, copies of the reports and records
+// into pinned host memory) on the batch stream.  Fill kernels stand in for
+// memsets; team launches are ordinary launches (their grid never exceeds the
+// resident workgroups, so the teams are co-resident on an otherwise idle GPU,
+// and every team wait is bounded).
+static int capture_step(nr_batch *b)
+{
+    const int count = (int)b->hnet.size();
+    nr::KArgs a;
+    fill_kargs(b, nr::kModeRelax, a);
+    hipStream_t st = b->stream;
+    HIP_TRY(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+    int e = nr::launch_strain_step(b->dnet.p, count, b->gstep.p, st);
+    if (!e) e = nr::launch_zero(b->run.p, sizeof(nr::NetRun) * b->run.n, st);
+    for (auto &ld : b->launches) {
+        if (!e) e = nr::launch_zero(ld->queues.p, sizeof(int) * ld->queues.n, st);
+        if (!e) e = nr::launch_zero(ld->partials.p, sizeof(unsigned long long) * ld->partials.n, st);
+        a.roles = ld->roles.p;
+        a.class_jobs = ld->class_jobs.p;
+        a.class_off = ld->class_off.p;
+        a.queues = ld->queues.p;
+        a.partials = ld->partials.p;
+        a.bar = ld->bar.p;
+        a.mailbox = ld->mailbox.p;
+        if (!e) e = nr::launch_relax(a, b->block, b->npt, b->method, ld->L.grid, b->lds, false, st);
+    }
+    if (!e) e = (int)hipMemcpyAsync(b->h_out, b->dout.p, sizeof(nr::NetOut) * count,
+                                    hipMemcpyDeviceToHost, st);
+    if (!e) e = (int)hipMemcpyAsync(b->h_net, b->dnet.p, sizeof(nr::NetDev) * count,
+                                    hipMemcpyDeviceToHost, st);
+    if (!e) e = (int)hipMemcpyAsync(b->h_err, b->err.p, sizeof(unsigned), hipMemcpyDeviceToHost, st);
+    hipGraph_t g = nullptr;
+    const hipError_t ec = hipStreamEndCapture(st, &g);
+    if (e) {
+        if (g) (void)hipGraphDestroy(g);
+        return hip_fail((hipError_t)e, "strain step capture");
+    }
+    if (ec != hipSuccess) return hip_fail(ec, "hipStreamEndCapture");
+    b->graph = g;
+    HIP_TRY(hipGraphInstantiate(&b->gexec, b->graph, nullptr, nullptr, 0));
+    return NR_OK;
+}
+
+int nr_batch_graph_capture(nr_batch *b, const double *xdisp, const double *ydisp,
+                           const nr_relax_params *params, int shared)
+{
+    if (!b || !xdisp || !ydisp || !params) return fail(NR_E_ARG, "null argument");
+    const int count = (int)b->hnet.size();
+    std::vector<double2> d(count);
+    for (int k = 0; k < count; k++) {
+        if (!std::isfinite(xdisp[k]) || !std::isfinite(ydisp[k]))
+            return fail(NR_E_ARG, "non-finite strain step");
+        if (b->hnet[k].height > 0 && ydisp[k] != 0.0)
+            return fail(NR_E_ARG, "compression is not defined in the Lees-Edwards box");
+        d[k] = make_double2(xdisp[k], ydisp[k]);
+    }
+    int rc = set_params(b, params, shared);
+    if (rc) return rc;
+    HIP_TRY(hipSetDevice(b->device));
+    b->drop_graph();
+    HIP_TRY(b->gstep.upload(d));
+    if (!b->h_out) {
+        HIP_TRY(hipHostMalloc((void **)&b->h_out, sizeof(nr::NetOut) * count, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void **)&b->h_net, sizeof(nr::NetDev) * count, hipHostMallocDefault));
+        HIP_TRY(hipHostMalloc((void **)&b->h_err, sizeof(unsigned), hipHostMallocDefault));
+    }
+    // pick the kernels' LDS size before capturing their launches
+    int per_cu = 0, n_cu = 0;
+    HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
+    return capture_step(b);
+}
+
+int nr_batch_graph_replay(nr_batch *b, nr_report *reports)
+{
+    if (!b || !b->gexec) return fail(NR_E_ARG, "no captured strain step");
+    HIP_TRY(hipSetDevice(b->device));
+    // the host's records (positions set, steps added since) go first
+    HIP_TRY(hipMemcpyAsync(b->dnet.p, b->hnet.data(), sizeof(nr::NetDev) * b->hnet.size(),
+                           hipMemcpyHostToDevice, b->stream));
+    HIP_TRY(hipEventRecord(b->ev0, b->stream));
+    HIP_TRY(hipGraphLaunch(b->gexec, b->stream));
+    HIP_TRY(hipEventRecord(b->ev1, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    float ms = 0.0f;
+    HIP_TRY(hipEventElapsedTime(&ms, b->ev0, b->ev1));
+    b->last_ms = ms;
+    std::memcpy(b->hnet.data(), b->h_net, sizeof(nr::NetDev) * b->hnet.size());
+    if (*b->h_err) {
+        (void)hipMemset(b->err.p, 0, sizeof(unsigned));
+        return fail(NR_E_TIMEOUT, "a team barrier exceeded its wait bound");
+    }
+    if (reports)
+        for (size_t k = 0; k < b->hnet.size(); k++) nr::to_report(b->h_out[k], reports[k]);
     return NR_OK;
 }
 

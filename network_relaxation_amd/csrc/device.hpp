@@ -128,5 +128,9 @@ inline size_t kernel_lds(int max_blob, int max_local, int max_fv)
 int launch_relax(const KArgs &a, int block, int npt, int method, int grid, size_t lds,
                  bool cooperative, void *stream);
 int max_resident_blocks(int block, int npt, size_t lds, int *per_cu, int *n_cu);
+// pend of network k += d[k] on the device (stream-ordered, graph-capturable).
+int launch_strain_step(NetDev *nets, int count, const double2 *d, void *stream);
+// Zeroes `bytes` (a multiple of 8) with a kernel (graph-capturable).
+int launch_zero(void *p, size_t bytes, void *stream);
 
 }  // namespace nr

@@ -1503,6 +1503,25 @@ __global__ __launch_bounds__(B) __attribute__((amdgpu_waves_per_eu(2))) void cg_
     kernel_body<B, NPT, true>(A);
 }
 
+// Affine steps of one strain step, applied on the device: network k's
+// pending step grows by d[k] (graph replays repeat it without host writes).
+__global__ void strain_step_kernel(NetDev *nets, int count, const double2 *d)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < count) {
+        nets[k].pend_x += d[k].x;
+        nets[k].pend_y += d[k].y;
+    }
+}
+
+// Zero fill of n 8-byte words (a kernel, so that graphs hold no memset node).
+__global__ void zero_words_kernel(unsigned long long *p, size_t n)
+{
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n;
+         i += (size_t)gridDim.x * blockDim.x)
+        p[i] = 0ull;
+}
+
 const void *kernel_for(int block, int npt, int method)
 {
     const bool cg = method == NR_METHOD_CG;
@@ -1558,6 +1577,24 @@ int max_resident_blocks(int block, int npt, size_t lds, int *per_cu, int *n_cu)
     *per_cu = nb;
     *n_cu = prop.multiProcessorCount;
     return 0;
+}
+
+int launch_strain_step(NetDev *nets, int count, const double2 *d, void *stream)
+{
+    if (count <= 0) return 0;
+    hipLaunchKernelGGL(strain_step_kernel, dim3((count + 255) / 256), dim3(256), 0,
+                       (hipStream_t)stream, nets, count, d);
+    return (int)hipGetLastError();
+}
+
+int launch_zero(void *p, size_t bytes, void *stream)
+{
+    const size_t n = bytes / 8;
+    if (n == 0) return 0;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(blocks), dim3(256), 0, (hipStream_t)stream,
+                       reinterpret_cast<unsigned long long *>(p), n);
+    return (int)hipGetLastError();
 }
 
 }  // namespace nr

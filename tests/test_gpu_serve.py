@@ -155,3 +155,29 @@ def test_multi_member_teams_alone_equal_in_mix():
         ra = alone.relax(prm)[0]
         assert _key(ra) == _key(rm[k]), k
         np.testing.assert_array_equal(alone.positions(0), mix.positions(k))
+
+
+def test_graph_step_replays_equal_eager_steps():
+    """A captured strain step (device affine step + relax) replayed three
+    times, with eager work and synchronisation between the replays, equals
+    the same three steps taken eagerly, bit for bit, every time."""
+    _say("case graph replay")
+    sizes = (16, 40, 72)
+    mk = lambda k: N.Network.lattice(sizes[k], 0.7, 90 + k, k != 1).triples()
+    nets_g = [mk(k) for k in range(3)]
+    nets_e = [mk(k) for k in range(3)]
+    prm = N.relax_params(dt_max=0.2, fcut=1e-10)
+    yr = np.array([_yrange(n) for n in nets_g])
+    xd = 0.01 * yr
+    yd = np.array([0.0, -0.002 * yr[1], 0.0])  # compression on the clamped network
+    g = N.Batch(nets_g, 1.0, 4e-3)
+    e = N.Batch(nets_e, 1.0, 4e-3)
+    g.graph_capture(xd, yd, prm)
+    for step in range(3):
+        rg = g.graph_replay()
+        for k in range(3):
+            e.strain(k, xd[k], yd[k])
+        re = e.relax(prm)
+        assert [_key(x) for x in rg] == [_key(x) for x in re], step
+        for k in range(3):  # eager work between replays
+            np.testing.assert_array_equal(g.positions(k), e.positions(k))
