@@ -186,6 +186,20 @@ int nr_batch_counts(const nr_batch *b, int index, int64_t *nodes,
                     int64_t *bonds, int64_t *triples);
 void nr_batch_destroy(nr_batch *b);
 
+/* Shape tuner (no counterpart in the reference, whose grid and block come
+ * from the -x/-y flags, large_strain_relax.cu:1059-1063): for these networks
+ * picks the workgroup/tile shape and the tiles-per-member rule.  The team
+ * rule comes from a cost model of the persistent launch; each candidate
+ * shape is timed by a probe relax of probe_steps FIRE steps per network.
+ * opts is read (precision, device, team; a tiles_per_member > 0 is kept) and
+ * written with the choice; *predicted_ms (may be NULL) estimates a relax of
+ * about 150 sqrt(N) steps per network.  Results depend on the shape only
+ * through reduction order, so use one choice for every batch whose results
+ * are compared. */
+int nr_autotune(int count, nr_network *const *nets, const double *ks,
+                const double *kb, int probe_steps, nr_batch_opts *opts,
+                double *predicted_ms);
+
 /* ---- the reference program's relax protocol ---------------------------- */
 
 /* relaxation_batch + do_relaxation_run (large_strain_relax.cu:792-1044):

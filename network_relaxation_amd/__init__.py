@@ -119,6 +119,7 @@ def lib():
         L.nr_batch_ramp.argtypes = [vp, C.c_int, dp, P(RelaxParams), C.c_int, P(Report)]
         L.nr_batch_graph_capture.argtypes = [vp, dp, dp, P(RelaxParams), C.c_int]
         L.nr_batch_graph_replay.argtypes = [vp, P(Report)]
+        L.nr_autotune.argtypes = [C.c_int, P(vp), dp, dp, C.c_int, P(BatchOpts), dp]
         L.nr_server_create.argtypes = [P(BatchOpts), P(ServerOpts), P(vp)]
         L.nr_server_clock.argtypes = [vp]
         L.nr_server_clock.restype = C.c_double
@@ -383,6 +384,20 @@ class Server:
         i = ServerInfo()
         _check(lib().nr_server_stats(self.h, C.byref(i)))
         return {f: getattr(i, f) for f, _ in ServerInfo._fields_}
+
+
+def autotune(networks, ks, kb, precision=PREC_FP64, probe_steps=200, device=0, team=0):
+    """nr_autotune: returns (shape kwargs for Batch/Server, predicted ms)."""
+    n = len(networks)
+    ks = np.broadcast_to(np.asarray(ks, np.float64), (n,)).copy()
+    kb = np.broadcast_to(np.asarray(kb, np.float64), (n,)).copy()
+    handles = (C.c_void_p * n)(*[x.h.value for x in networks])
+    opts = BatchOpts(precision, 0, 0, team, device, 0, 0.0)
+    pred = C.c_double()
+    _check(lib().nr_autotune(n, handles, _dp(ks), _dp(kb), probe_steps, C.byref(opts),
+                             C.byref(pred)))
+    return dict(block=opts.block, tile_nodes=opts.tile_nodes,
+                tiles_per_member=opts.tiles_per_member), pred.value
 
 
 def relaxation_batch(batch_file, strain_mode=True, precision=PREC_FP64, tile_nodes=0):

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <limits>
 #include <memory>
 #include <string>
 #include <vector>
@@ -73,6 +74,7 @@ struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
     int npt = 1, block = 256, team_cap = 256, tiles_per_member = 4;
+    int resident = 0;             // workgroups the relax kernel keeps resident
     int method = NR_METHOD_FIRE;  // of the last relax call (one method per call)
     size_t lds = 0;
     std::vector<std::unique_ptr<LaunchDev>> launches;
@@ -247,6 +249,42 @@ static int choose_shapes(nr_batch *b, const nr_batch_opts *opts, int64_t *lds_bu
     return NR_OK;
 }
 
+// Teams: each network gets a power-of-two team of workgroups (a function of
+// its tile count and the options only, so its arithmetic does not depend on
+// the rest of the batch); the resident workgroups are shared between the
+// team-size classes to minimise the predicted finish time.
+static int set_team_layout(nr_batch *b, int tiles_per_member)
+{
+    const nr::Plan &P = b->plan;
+    const int count = (int)P.nets.size();
+    b->tiles_per_member = tiles_per_member;
+    std::vector<nr::JobSpec> jobs(count);
+    for (int k = 0; k < count; k++)
+        jobs[k] = {nr::team_size(P.nets[k].n_tiles, b->team_cap, b->tiles_per_member),
+                   P.nets[k].n_tiles, nr::default_steps(P.nets[k].n_nodes)};
+    std::vector<nr::TeamLayout> layouts = nr::plan_launches(jobs, b->resident);
+    b->launches.clear();
+    b->team_of_net.assign(count, 1);
+    for (nr::TeamLayout &L : layouts) {
+        auto ld = std::make_unique<LaunchDev>();
+        for (int k = 0; k < count; k++)
+            if (L.team_of_net[k] > 0) b->team_of_net[k] = L.team_of_net[k];
+        HIP_TRY(ld->roles.upload(L.roles));
+        HIP_TRY(ld->class_jobs.upload(L.class_jobs));
+        HIP_TRY(ld->class_off.upload(L.class_off));
+        HIP_TRY(ld->queues.alloc((size_t)32 * (L.class_off.size() - 1)));
+        HIP_TRY(ld->mailbox.alloc((size_t)32 * std::max(1, L.n_teams)));
+        HIP_TRY(ld->partials.alloc((size_t)nr::kPartWords * std::max(1, L.n_parts)));
+        HIP_TRY(ld->bar.alloc((size_t)nr::kBarStride * std::max(1, L.n_teams)));
+        HIP_TRY(hipMemset(ld->bar.p, 0, sizeof(unsigned long long) * nr::kBarStride *
+                                             std::max(1, L.n_teams)));
+        ld->L = std::move(L);
+        b->launches.push_back(std::move(ld));
+    }
+    b->drop_graph();  // a captured step refers to the old tables
+    return NR_OK;
+}
+
 int nr_batch_create(int count, nr_network *const *nets, const double *ks, const double *kb,
                     const nr_batch_opts *opts, nr_batch **out)
 {
@@ -275,42 +313,14 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     b->lds = (size_t)(nr::kCacheBytes + P.lds_bytes());
     if (b->lds > 160 * 1024) return fail(NR_E_TOPO, "tile working set exceeds LDS");
 
-    // teams: each network gets a power-of-two team of workgroups (a function
-    // of its tile count and the options only, so its arithmetic does not
-    // depend on the rest of the batch); the resident workgroups are shared
-    // between the team-size classes to minimise the predicted finish time
     int per_cu = 0, n_cu = 0;
     HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
     if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
-    const int resident = per_cu * n_cu;
-    b->team_cap = opts && opts->team > 0 ? opts->team : 256;
-    b->tiles_per_member =
-        opts && opts->tiles_per_member > 0 ? opts->tiles_per_member : kDefaultTilesPerMember;
-    if (b->team_cap > resident) b->team_cap = resident;
-    {
-        std::vector<nr::JobSpec> jobs(count);
-        for (int k = 0; k < count; k++)
-            jobs[k] = {nr::team_size(P.nets[k].n_tiles, b->team_cap, b->tiles_per_member),
-                       P.nets[k].n_tiles, nr::default_steps(P.nets[k].n_nodes)};
-        std::vector<nr::TeamLayout> layouts = nr::plan_launches(jobs, resident);
-        b->team_of_net.assign(count, 1);
-        for (nr::TeamLayout &L : layouts) {
-            auto ld = std::make_unique<LaunchDev>();
-            for (int k = 0; k < count; k++)
-                if (L.team_of_net[k] > 0) b->team_of_net[k] = L.team_of_net[k];
-            HIP_TRY(ld->roles.upload(L.roles));
-            HIP_TRY(ld->class_jobs.upload(L.class_jobs));
-            HIP_TRY(ld->class_off.upload(L.class_off));
-            HIP_TRY(ld->queues.alloc((size_t)32 * (L.class_off.size() - 1)));
-            HIP_TRY(ld->mailbox.alloc((size_t)32 * std::max(1, L.n_teams)));
-            HIP_TRY(ld->partials.alloc((size_t)nr::kPartWords * std::max(1, L.n_parts)));
-            HIP_TRY(ld->bar.alloc((size_t)nr::kBarStride * std::max(1, L.n_teams)));
-            HIP_TRY(hipMemset(ld->bar.p, 0, sizeof(unsigned long long) * nr::kBarStride *
-                                                 std::max(1, L.n_teams)));
-            ld->L = std::move(L);
-            b->launches.push_back(std::move(ld));
-        }
-    }
+    b->resident = per_cu * n_cu;
+    b->team_cap = std::min(opts && opts->team > 0 ? opts->team : 256, b->resident);
+    rc = set_team_layout(b.get(), opts && opts->tiles_per_member > 0 ? opts->tiles_per_member
+                                                               : kDefaultTilesPerMember);
+    if (rc) return rc;
 
     // uploads
     HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
@@ -734,5 +744,76 @@ int nr_batch_counts(const nr_batch *b, int index, int64_t *nodes, int64_t *bonds
 }
 
 void nr_batch_destroy(nr_batch *b) { delete b; }
+
+// Shape tuner.  For each candidate workgroup/tile shape the cost model picks
+// the tiles-per-member rule (teams.cpp), then a probe relax of probe_steps
+// FIRE steps per network (no convergence test) measures that shape's cost
+// per model unit on this GPU; the shape with the lowest predicted time for
+// a relax of about 150 sqrt(N) steps per network wins.
+int nr_autotune(int count, nr_network *const *nets, const double *ks, const double *kb,
+                int probe_steps, nr_batch_opts *opts, double *predicted_ms)
+{
+    if (count <= 0 || !nets || !ks || !kb || !opts || probe_steps < 1)
+        return fail(NR_E_ARG, "invalid arguments");
+    struct Shape {
+        int block, tile;
+    };
+    std::vector<Shape> shapes = {{256, 256}, {512, 512}};
+    if (opts->precision == NR_PREC_FP64) shapes.push_back({256, 512});
+    const std::vector<int> cands = {1, 2, 3, 4, 5, 6, 8, 12, 16};
+    double best = std::numeric_limits<double>::infinity();
+    nr_batch_opts best_opts = *opts;
+    for (const Shape &sh : shapes) {
+        nr_batch_opts o = *opts;
+        o.block = sh.block;
+        o.tile_nodes = sh.tile;
+        o.tiles_per_member = 1;
+        nr_batch *raw = nullptr;
+        int rc = nr_batch_create(count, nets, ks, kb, &o, &raw);
+        if (rc == NR_E_TOPO || rc == NR_E_ARG) continue;  // shape cannot hold these networks
+        if (rc) return rc;
+        std::unique_ptr<nr_batch> b(raw);
+        std::vector<int> tiles(count);
+        std::vector<double> full(count), probe(count, (double)probe_steps);
+        for (int k = 0; k < count; k++) {
+            tiles[k] = b->plan.nets[k].n_tiles;
+            full[k] = 150.0 * nr::default_steps(b->plan.nets[k].n_nodes);
+        }
+        const int tpm = opts->tiles_per_member > 0
+                            ? opts->tiles_per_member
+                            : nr::tune_tiles_per_member(tiles, full, b->team_cap, b->resident,
+                                                        cands, nullptr);
+        rc = set_team_layout(b.get(), tpm);
+        if (rc) return rc;
+        nr_relax_params prm;
+        std::memset(&prm, 0, sizeof(prm));
+        prm.alpha0 = 0.1;
+        prm.f_alpha = 0.99;
+        prm.dt_max = 0.1;
+        prm.f_inc = 1.1;
+        prm.f_dec = 0.5;
+        prm.nmin = 5;
+        prm.mass = 1.0;
+        prm.fcut = 0.0;  // never met: every network takes probe_steps steps
+        prm.max_steps = probe_steps;
+        prm.criterion = NR_CRIT_MAX;
+        prm.method = NR_METHOD_FIRE;
+        rc = nr_batch_relax(b.get(), &prm, 1, nullptr);
+        if (rc) return rc;
+        const double units_probe =
+            nr::predict_time(tiles, probe, b->team_cap, b->resident, tpm);
+        const double units_full = nr::predict_time(tiles, full, b->team_cap, b->resident, tpm);
+        const double t = b->last_ms / units_probe * units_full;
+        if (t < best) {
+            best = t;
+            best_opts = o;
+            best_opts.tiles_per_member = tpm;
+        }
+    }
+    if (!std::isfinite(best)) return fail(NR_E_TOPO, "no shape holds these networks");
+    *opts = best_opts;
+    if (predicted_ms) *predicted_ms = best;
+    return NR_OK;
+}
 
 }  // extern "C"

@@ -119,6 +119,11 @@ int build_plan(const std::vector<const Network *> &nets,
     for (size_t k = 0; k < nets.size(); k++) {
         const Network &net = *nets[k];
         const bool use_b = ks[k] > 0, use_t = kb[k] > 0;
+        if (net.n >= (1 << 27)) {
+            // the kernel addresses a network's nodes with 32-bit byte offsets
+            err = "a network has 2^27 nodes or more";
+            return NR_E_TOPO;
+        }
         if (use_t && !net.triples_built) {
             err = "bending stiffness > 0 but triples were not built";
             return NR_E_ARG;

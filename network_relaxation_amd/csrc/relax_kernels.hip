@@ -93,17 +93,46 @@ __device__ inline double2 ld_state(bool sh, const double2 *p)
     return *p;
 }
 
+// A shared double2 goes out as ONE 16-byte store with the agent-scope cache
+// policy (sc1, what an agent-scope relaxed atomic store of 8 bytes uses):
+// two 8-byte stores to the halves of 16-byte slots would write every 32-byte
+// sector twice.  The store has no outputs; the drain before a record is
+// published waits for it (vmcnt) like any other store.
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ inline void st_state(bool sh, double2 *p, double2 v)
 {
     if (sh) {
-        unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
-        __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
+        const unsigned long long a = (unsigned long long)__double_as_longlong(v.x);
+        const unsigned long long b = (unsigned long long)__double_as_longlong(v.y);
+        u32x4 w;
+        w.x = (unsigned)a;
+        w.y = (unsigned)(a >> 32);
+        w.z = (unsigned)b;
+        w.w = (unsigned)(b >> 32);
+        asm volatile("global_store_dwordx4 %0, %1, off sc1" : : "v"(p), "v"(w) : "memory");
     } else {
         *p = v;
     }
+}
+
+// A shared double2 of the relax state comes in as ONE 16-byte load with the
+// agent-scope cache policy (sc1), through a buffer descriptor over the
+// network's node range (offsets relative to the network stay below 4 GB:
+// networks have fewer than 2^27 nodes; reads past the range return zero).
+constexpr int kBufWord3 = 0x00020000;  // raw buffer, gfx9 family
+constexpr int kPolicySc1 = 16;         // cache policy bit sc1 (agent scope)
+__device__ inline double2 ld_node(bool sh, const double2 *arr, const NetCtx &C, int gid)
+{
+    if (sh) {
+        const __amdgpu_buffer_rsrc_t r = __builtin_amdgcn_make_buffer_rsrc(
+            const_cast<double2 *>(arr + C.node_base), (short)0, C.n_nodes * 16, kBufWord3);
+        const u32x4 w =
+            __builtin_amdgcn_raw_buffer_load_b128(r, (gid - C.node_base) * 16, 0, kPolicySc1);
+        return make_double2(
+            __longlong_as_double((long long)(((unsigned long long)w.y << 32) | w.x)),
+            __longlong_as_double((long long)(((unsigned long long)w.w << 32) | w.z)));
+    }
+    return arr[gid];
 }
 
 __device__ inline float2 ld_state32(bool sh, const float2 *p)
@@ -883,11 +912,11 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                     const float2 v32 = ld_state32(sh, A.v32[S.s] + gid);
                     vo[j] = make_double2((double)v32.x, (double)v32.y);
                 } else {
-                    xo[j] = ld_state(sh, X + gid);
+                    xo[j] = ld_node(sh, X, C, gid);
                 }
                 if (KIND == kPassStep || KIND == kPassCgTrial) {
-                    vo[j] = ld_state(sh, V + gid);
-                    fo[j] = ld_state(sh, F + gid);
+                    vo[j] = ld_node(sh, V, C, gid);
+                    fo[j] = ld_node(sh, F, C, gid);
                 }
                 if (KIND != kPassEnergy) mk[j] = A.mask[gid];
             }
@@ -900,10 +929,10 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
             if (KIND == kPassFuse) {
                 hx = fuse_position(sh, A, S, X, gid);
             } else {
-                hx = ld_state(sh, X + gid);
+                hx = ld_node(sh, X, C, gid);
                 if (KIND == kPassStep || KIND == kPassCgTrial) {
-                    hv = ld_state(sh, V + gid);
-                    hf = ld_state(sh, F + gid);
+                    hv = ld_node(sh, V, C, gid);
+                    hf = ld_node(sh, F, C, gid);
                 }
             }
         }
@@ -965,14 +994,14 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
             const int gid = hid != nullptr ? hid[k - td.n_own] : bb.halo[k - td.n_own];
             double2 dummy, xn;
             if (KIND == kPassEnergy)
-                xn = ld_state(sh, X + gid);
+                xn = ld_node(sh, X, C, gid);
             else if (KIND == kPassInit)
-                xn = advance<kPassInit>(ld_state(sh, X + gid), zero, zero, S, C, dummy);
+                xn = advance<kPassInit>(ld_node(sh, X, C, gid), zero, zero, S, C, dummy);
             else if (KIND == kPassFuse)
                 xn = fuse_position(sh, A, S, X, gid);
             else
-                xn = advance<KIND>(ld_state(sh, X + gid), ld_state(sh, V + gid),
-                                   ld_state(sh, F + gid), S, C, dummy);
+                xn = advance<KIND>(ld_node(sh, X, C, gid), ld_node(sh, V, C, gid),
+                                   ld_node(sh, F, C, gid), S, C, dummy);
             L.pos[k] = xn;
         }
         __syncthreads();

@@ -156,6 +156,17 @@ std::vector<TeamLayout> plan_launches(const std::vector<JobSpec> &jobs, int resi
     return out;
 }
 
+double predict_time(const std::vector<int> &n_tiles, const std::vector<double> &steps, int cap,
+                    int resident, int tiles_per_member)
+{
+    std::vector<JobSpec> jobs(n_tiles.size());
+    for (size_t k = 0; k < n_tiles.size(); k++)
+        jobs[k] = {team_size(n_tiles[k], cap, tiles_per_member), n_tiles[k], steps[k]};
+    double t = 0.0;
+    for (const TeamLayout &L : plan_launches(jobs, resident)) t += L.makespan;
+    return t;
+}
+
 int tune_tiles_per_member(const std::vector<int> &n_tiles, const std::vector<double> &steps,
                           int cap, int resident, const std::vector<int> &candidates,
                           double *predicted)
@@ -163,11 +174,7 @@ int tune_tiles_per_member(const std::vector<int> &n_tiles, const std::vector<dou
     int best = candidates.empty() ? 1 : candidates[0];
     double best_t = std::numeric_limits<double>::infinity();
     for (int r : candidates) {
-        std::vector<JobSpec> jobs(n_tiles.size());
-        for (size_t k = 0; k < n_tiles.size(); k++)
-            jobs[k] = {team_size(n_tiles[k], cap, r), n_tiles[k], steps[k]};
-        double t = 0.0;
-        for (const TeamLayout &L : plan_launches(jobs, resident)) t += L.makespan;
+        const double t = predict_time(n_tiles, steps, cap, resident, r);
         if (t < best_t) {
             best_t = t;
             best = r;

@@ -52,6 +52,11 @@ TeamLayout plan_teams(const std::vector<JobSpec> &jobs, int resident);
 // several launches, largest teams first.  Job lists hold batch indices.
 std::vector<TeamLayout> plan_launches(const std::vector<JobSpec> &jobs, int resident);
 
+// Predicted time (cost-model units: plan_launches makespans added over
+// launches) of relaxing the networks with this tiles-per-member rule.
+double predict_time(const std::vector<int> &n_tiles, const std::vector<double> &steps, int cap,
+                    int resident, int tiles_per_member);
+
 // Tiles per member that minimises the predicted time of the job set over
 // `candidates` (plan_launches makespans added over launches).
 int tune_tiles_per_member(const std::vector<int> &n_tiles, const std::vector<double> &steps,

@@ -181,3 +181,23 @@ def test_graph_step_replays_equal_eager_steps():
         assert [_key(x) for x in rg] == [_key(x) for x in re], step
         for k in range(3):  # eager work between replays
             np.testing.assert_array_equal(g.positions(k), e.positions(k))
+
+
+def test_autotune_returns_a_working_shape():
+    """The tuner's choice is a valid shape whose relax reaches the same
+    minima (team sizes change only the reduction order)."""
+    _say("case autotune")
+    sizes = (16, 48, 96, 64, 24)
+    nets = [N.Network.lattice(L, 0.66, 200 + L, True).triples() for L in sizes]
+    shape, pred = N.autotune(nets, 1.0, 2e-3, probe_steps=50)
+    assert shape["block"] in (256, 512) and shape["tiles_per_member"] >= 1 and pred > 0
+    prm = N.relax_params(dt_max=0.2, fcut=1e-10)
+    tuned = N.Batch(nets, 1.0, 2e-3, **shape)
+    base = N.Batch(nets, 1.0, 2e-3)
+    tuned.shear(-1, 0.01)
+    base.shear(-1, 0.01)
+    rt, rb = tuned.relax(prm), base.relax(prm)
+    for a, b in zip(rt, rb):
+        assert a["status"] == N.CONVERGED
+        ea, eb = a["es"] + a["eb"], b["es"] + b["eb"]
+        assert abs(ea - eb) <= 1e-12 * eb
