@@ -95,9 +95,24 @@ def main():
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "single", "ramp", "large"],
+                    help="batched: the headline (512 uneven networks); single: one "
+                         "L=32 network vs the oracle; ramp: L=512, 100 shear increments, "
+                         "FIRE and CG; large: L=2048 mixed vs FP64 per increment")
+    ap.add_argument("--increments", type=int, default=0,
+                    help="ramp/large: increments to run (0 = 100 / 2)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
+        return
+    if args.workload != "batched":
+        if args.gpus != 1:
+            print("bench.py runs on one GPU only (use --gpus 1)", file=sys.stderr)
+            sys.exit(2)
+        import network_relaxation_amd as N
+        leg = {"single": single_leg, "ramp": ramp_leg, "large": large_leg}[args.workload]
+        print(json.dumps(leg(N, args)), flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -234,6 +249,114 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def single_leg(N, args):
+    """One L=32 Lees-Edwards network (p 0.65, kappa 1e-3, affine shear 0.01)
+    relaxed by FIRE to |F|max < 1e-10 (dt_max 0.1), against the CPU oracle."""
+    import oracle as O
+    fire = dict(FIRE, dt_max=0.1)
+    n = N.Network.lattice(32, 0.65, 1, True).triples()
+    b = N.Batch([n], 1.0, 1e-3, tiles_per_member=1)
+    prm = N.relax_params(fcut=1e-10, criterion=N.CRIT_MAX, **fire)
+    times = []
+    for _ in range(3):
+        b.reset()
+        b.shear(0, 0.01)
+        t = time.perf_counter()
+        r = b.relax(prm)[0]
+        times.append(time.perf_counter() - t)
+    o = O.Net.lattice(32, 0.65, 1, True).triples()
+    pos = o.pos()
+    o.shear(pos, 0.01)
+    t = time.perf_counter()
+    ro = o.fire(1.0, 1e-3, pos, fp=(fire["alpha0"], fire["f_alpha"], fire["dt_max"],
+                                    fire["f_inc"], fire["f_dec"]), fcut=1e-10)
+    t_cpu = time.perf_counter() - t
+    e, eo = r["es"] + r["eb"], ro["es"] + ro["eb"]
+    return {"metric": "config 1: time to relax one L=32 network, FP64 FIRE, |F|max<1e-10",
+            "value": min(times) * 1e3, "unit": "ms", "higher_is_better": False,
+            "steps": r["steps"], "oracle_steps": ro["steps"], "fmax": r["fmax"],
+            "energy_rel_vs_oracle": abs(e - eo) / eo, "kernel_ms": b.last_ms(),
+            "cpu_oracle_ms": t_cpu * 1e3, "cpu_cores": 1}
+
+
+def ramp_leg(N, args):
+    """L=512 (p 0.6, kappa 1e-4) Lees-Edwards network, shear 0 -> 0.5 in 100
+    increments of 0.005, each relaxed to |F|max < 1e-10 from the previous
+    state, FIRE and CG, the whole ramp in one device launch (nr_batch_ramp)."""
+    ninc = args.increments or 100
+    n = N.Network.lattice(512, 0.6, 2, True).triples()
+    nodes, edges, tris = n.info().n_nodes, n.info().n_bonds, n.info().n_triples
+    incs = np.array([(0.005 * n.info().height, 0.0)] * ninc)
+    out = {"metric": f"config 2: node-updates/s, L=512 shear ramp of {ninc} x 0.005",
+           "unit": "node-updates/s", "higher_is_better": True,
+           "nodes": nodes, "bonds": edges, "triples": tris}
+    for name, method in (("fire", N.METHOD_FIRE), ("cg", N.METHOD_CG)):
+        b = N.Batch([n], 1.0, 1e-4)
+        prm = N.relax_params(fcut=1e-10, criterion=N.CRIT_MAX, method=method,
+                             max_steps=1000000, **dict(FIRE, dt_max=0.1))
+        t = time.perf_counter()
+        reps, kms = [], 0.0
+        for c0 in range(0, ninc, 10):  # ten increments per launch, a progress line each
+            reps += b.ramp(incs[c0:c0 + 10], prm)[0]
+            kms += b.last_ms()
+            progress(f"ramp {name}: {len(reps)}/{ninc} increments, "
+                     f"{sum(r['steps'] for r in reps)} steps")
+        wall = time.perf_counter() - t
+        steps = sum(r["steps"] for r in reps)
+        n_, e_, t_ = b.counts(0)
+        byt = steps * alg_bytes(n_, e_, t_, per_triple_kappa=False)
+        out[name] = {"node_updates_per_s": nodes * steps / (kms * 1e-3),
+                     "steps": steps, "kernel_ms": kms, "wall_s": wall,
+                     "converged": sum(r["status"] == N.CONVERGED for r in reps),
+                     "hbm_alg_GBs": byt / (kms * 1e-3) / 1e9,
+                     "hbm_frac": byt / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                     "final_energy": reps[-1]["es"] + reps[-1]["eb"],
+                     "final_shear_stress": reps[-1]["sxy"]}
+        del b
+    out["value"] = out["fire"]["node_updates_per_s"]
+    e1, e2 = out["fire"]["final_energy"], out["cg"]["final_energy"]
+    out["fire_vs_cg_final_energy_rel"] = abs(e1 - e2) / e1
+    return out
+
+
+def large_leg(N, args):
+    """L=2048 (p 0.62, per-fiber kappa log-U[1e-5,1e-2]) Lees-Edwards network,
+    shear increments of 0.005 from the unstrained state, mixed precision
+    (FP32 stage + FP64 polish) against FP64, per increment: energy and shear
+    stress relative error, time."""
+    ninc = args.increments or 2
+    n = N.Network.lattice(2048, 0.62, 3, True).triples().fiber_kappa(1e-5, 1e-2, 3)
+    progress("large: network built")
+    incs = np.array([(0.005 * n.info().height, 0.0)] * ninc)
+    prm = N.relax_params(fcut=1e-10, criterion=N.CRIT_MAX, max_steps=3000000,
+                         **dict(FIRE, dt_max=0.1))
+    res = {}
+    for name, prec in (("mixed", N.PREC_MIXED), ("fp64", N.PREC_FP64)):
+        b = N.Batch([n], 1.0, 1.0, precision=prec)
+        t = time.perf_counter()
+        reps, kms = [], 0.0
+        for i in range(ninc):
+            reps += b.ramp(incs[i:i + 1], prm)[0]
+            kms += b.last_ms()
+            progress(f"large {name}: increment {i + 1}/{ninc}, {reps[-1]['steps']} steps, "
+                     f"{b.last_ms() / 1e3:.1f} s")
+        res[name] = (reps, kms, time.perf_counter() - t)
+        del b
+    rm, rf = res["mixed"][0], res["fp64"][0]
+    rows = []
+    for i in range(ninc):
+        em, ef = rm[i]["es"] + rm[i]["eb"], rf[i]["es"] + rf[i]["eb"]
+        rows.append({"gamma": 0.005 * (i + 1), "energy_rel": abs(em - ef) / ef,
+                     "stress_rel": abs(rm[i]["sxy"] - rf[i]["sxy"]) / abs(rf[i]["sxy"]),
+                     "steps_mixed": rm[i]["steps"], "fp64_polish_steps": rm[i]["fp64_steps"],
+                     "steps_fp64": rf[i]["steps"]})
+    return {"metric": f"config 3: L=2048 mixed vs FP64, {ninc} shear increments of 0.005",
+            "unit": "s", "value": res["mixed"][1] * 1e-3, "higher_is_better": False,
+            "fp64_s": res["fp64"][1] * 1e-3, "nodes": n.info().n_nodes,
+            "max_energy_rel": max(r["energy_rel"] for r in rows),
+            "max_stress_rel": max(r["stress_rel"] for r in rows), "increments": rows}
 
 
 def served_leg(N, args):

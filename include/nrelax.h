@@ -86,6 +86,10 @@ typedef struct {
     int pad;
     double t_start;    /* device clock (s) when this network's relax began */
     double t_done;     /* device clock (s) when it finished */
+    double shear_stress; /* sigma_xy = (dE/dgamma of an affine shear) / box
+                            area, box = width x height (Lees-Edwards) or
+                            width x (maxy - miny); an extension, the
+                            reference reports no stress */
 } nr_report;
 
 typedef struct {

@@ -40,12 +40,14 @@ class Report(C.Structure):
     _fields_ = [("rms_force", C.c_double), ("max_force", C.c_double),
                 ("e_stretch", C.c_double), ("e_bend", C.c_double),
                 ("steps", C.c_int), ("status", C.c_int), ("fp64_steps", C.c_int),
-                ("pad", C.c_int), ("t_start", C.c_double), ("t_done", C.c_double)]
+                ("pad", C.c_int), ("t_start", C.c_double), ("t_done", C.c_double),
+                ("shear_stress", C.c_double)]
 
     def as_dict(self):
         return dict(rms=self.rms_force, fmax=self.max_force, es=self.e_stretch,
                     eb=self.e_bend, steps=self.steps, status=self.status,
-                    fp64_steps=self.fp64_steps, t_start=self.t_start, t_done=self.t_done)
+                    fp64_steps=self.fp64_steps, t_start=self.t_start, t_done=self.t_done,
+                    sxy=self.shear_stress)
 
 
 class BatchOpts(C.Structure):

@@ -45,6 +45,7 @@ void to_report(const NetOut &o, nr_report &r)
     r.pad = 0;
     r.t_start = (double)o.t_start * 1e-8;
     r.t_done = (double)o.t_done * 1e-8;
+    r.shear_stress = o.sxy;
 }
 
 }  // namespace nr
@@ -58,6 +59,10 @@ using nr::hip_fail;
 // Default tiles per team member: the best measured on MI355X for networks
 // of 1 to 150 tiles of 256 nodes.
 constexpr int kDefaultTilesPerMember = 4;
+// Default largest team: every resident workgroup of an MI355X at two 256-thread
+// workgroups per CU (a fixed number, so that team sizes do not depend on the
+// batch; the resident count still bounds it).
+constexpr int kDefaultTeamCap = 512;
 
 // Device tables of one persistent launch.
 struct LaunchDev {
@@ -317,7 +322,7 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
     if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
     b->resident = per_cu * n_cu;
-    b->team_cap = std::min(opts && opts->team > 0 ? opts->team : 256, b->resident);
+    b->team_cap = std::min(opts && opts->team > 0 ? opts->team : kDefaultTeamCap, b->resident);
     rc = set_team_layout(b.get(), opts && opts->tiles_per_member > 0 ? opts->tiles_per_member
                                                                : kDefaultTilesPerMember);
     if (rc) return rc;

@@ -50,6 +50,7 @@ struct NetOut {
     double rms, fmax, es, eb;
     int steps, status, fp64_steps, pad;
     long long t_start, t_done;  // 100 MHz device clock at job start / finish
+    double sxy;                 // shear stress: shear virial / box area
 };
 
 enum KernelMode { kModeRelax = 0, kModeEval = 1 };

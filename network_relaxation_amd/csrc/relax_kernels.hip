@@ -1014,6 +1014,9 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                     const double d = sqrt(fma(rx, rx, ry * ry));
                     const double dl = d - bb.bl[e];
                     acc.a += 0.5 * C.ks * dl * dl;
+                    // shear virial dE/dgamma: ks (1 - L/|r|) r_x r_y
+                    const double s = C.ks * (1.0 - bb.bl[e] * (1.0 / d));
+                    acc.c += s * rx * ry;
                 }
             } else {
                 // :401-404: f = ks (1 - L/|r|) r on the first node
@@ -1033,7 +1036,14 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
             const double dth = atan2(-cr, -dt);
             const double kk = bb.tk[t];
             if (KIND == kPassEnergy) {
-                if (h < td.n_own) acc.b += 0.5 * kk * dth * dth;
+                if (h < td.n_own) {
+                    acc.b += 0.5 * kk * dth * dth;
+                    // shear virial: -(F_a,x a_y + F_b,x b_y)
+                    const double g = kk * dth;
+                    const double f2x = g * (1.0 / fma(ax, ax, ay * ay)) * -ay;
+                    const double f3x = g * (1.0 / fma(bx, bx, by * by)) * by;
+                    acc.c -= f2x * ay + f3x * by;
+                }
             } else {
                 // :453-468 force on end a, end b and the hinge
                 const double ia = 1.0 / fma(ax, ax, ay * ay);
@@ -1418,6 +1428,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             o.fmax = sqrt(fmax2);
             o.es = C.ks > 0 ? e.a : 0.0;
             o.eb = C.kb > 0 ? e.b : 0.0;
+            o.sxy = e.c / (C.W * C.yrange);
             o.steps = step;
             o.status = status;
             o.fp64_steps = step - steps32;

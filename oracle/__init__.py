@@ -215,6 +215,13 @@ class Net:
         lib().or_energies(C.byref(self.s), ks, kb, _dp(pos), C.byref(es), C.byref(eb))
         return es.value, eb.value
 
+    def shear_virial(self, ks, kb, pos):
+        """dE/dgamma under an affine shear of the element geometry."""
+        f = lib().or_shear_virial
+        f.restype = C.c_double
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        return f(C.byref(self.s), C.c_double(ks), C.c_double(kb), _dp(pos))
+
     def affine(self, pos, xdisp, ydisp, miny=None, yrange=None):
         """In-place affine step of affine_displace; in Lees-Edwards mode a shear
         step also advances the image shift."""

@@ -463,6 +463,51 @@ void or_energies(const or_net *net, double ks, double kb, const double *pos,
     *eb = u;
 }
 
+/* Shear virial W = dE/dgamma under the affine shear x += gamma (y - y0) of
+ * every element's geometry (extension: the reference reports no stress).
+ * Bond: ks (1 - L/|r|) r_x r_y.  Triple with arms a = r_a - r_h, b = r_c - r_h
+ * and end forces f_a, f_c as in or_forces: -(f_a,x a_y + f_c,x b_y).  The
+ * shear stress is W over the box area. */
+double or_shear_virial(const or_net *net, double ks, double kb, const double *pos)
+{
+    int b, t;
+    double w = 0.0;
+    if (ks > 0) {
+        for (b = 0; b < net->e; b++) {
+            int p = net->bond[2 * b], q = net->bond[2 * b + 1];
+            double ox, oy, rx, ry, d2, inv, s;
+            or_image(net, net->bwrap[2 * b], net->bwrap[2 * b + 1], &ox, &oy);
+            rx = pos[2 * q] + ox - pos[2 * p];
+            ry = pos[2 * q + 1] + oy - pos[2 * p + 1];
+            d2 = fma(rx, rx, ry * ry);
+            inv = 1.0 / sqrt(d2);
+            s = ks * (1.0 - net->rest[b] * inv);
+            w += s * rx * ry;
+        }
+    }
+    if (kb > 0) {
+        for (t = 0; t < net->t; t++) {
+            int h = net->tri[3 * t], a = net->tri[3 * t + 1], c = net->tri[3 * t + 2];
+            double oxa, oya, oxb, oyb, ax, ay, bx, by, cr, dt, dth, k, g, f2x, f3x;
+            or_image(net, net->twrap[4 * t], net->twrap[4 * t + 1], &oxa, &oya);
+            or_image(net, net->twrap[4 * t + 2], net->twrap[4 * t + 3], &oxb, &oyb);
+            ax = pos[2 * a] + oxa - pos[2 * h];
+            ay = pos[2 * a + 1] + oya - pos[2 * h + 1];
+            bx = pos[2 * c] + oxb - pos[2 * h];
+            by = pos[2 * c + 1] + oyb - pos[2 * h + 1];
+            cr = fma(ax, by, -(ay * bx));
+            dt = fma(ax, bx, ay * by);
+            dth = atan2(-cr, -dt);
+            k = net->tkb ? net->tkb[t] : kb;
+            g = k * dth;
+            f2x = g * (1.0 / fma(ax, ax, ay * ay)) * -ay;
+            f3x = g * (1.0 / fma(bx, bx, by * by)) * by;
+            w -= f2x * ay + f3x * by;
+        }
+    }
+    return w;
+}
+
 /* affine_displace :527-537 over the N nodes (the reference launches 2N
  * threads with a `>` guard and so writes past the array; only the in-bounds
  * part is reproduced). */

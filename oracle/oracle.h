@@ -76,6 +76,9 @@ void or_energies(const or_net *net, double ks, double kb, const double *pos,
                  double *es, double *eb);
 void or_affine(const or_net *net, double *pos, double miny, double yrange,
                double xdisp, double ydisp);
+/* Shear virial dE/dgamma (affine shear of the element geometry); the shear
+ * stress is this over the box area.  Extension: no reference counterpart. */
+double or_shear_virial(const or_net *net, double ks, double kb, const double *pos);
 
 /* FIRE parameters: alpha0, f_alpha, dt_max, f_inc, f_dec */
 #define OR_CRIT_RMS 0

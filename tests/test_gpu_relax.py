@@ -258,3 +258,24 @@ def test_coincident_bonded_nodes_report_nonfinite():
     b.set_positions(0, pos)
     r = b.relax(N.relax_params(fcut=1e-12, max_steps=1000))[0]
     assert r["status"] == N.NONFINITE and r["steps"] < 1000
+
+
+@pytest.mark.parametrize("le,precision", [(True, N.PREC_FP64), (False, N.PREC_FP64),
+                                          (True, N.PREC_MIXED)])
+def test_shear_stress_matches_oracle_virial(le, precision):
+    """sigma_xy of the relaxed state = the oracle's shear virial (dE/dgamma,
+    pinned by finite differences) over the box area, to 1e-10 relative."""
+    _say(f"case shear stress le={le} precision={precision}")
+    o, n = _pair(24, 0.7, 31, le, 1.0, kappa=(1e-4, 1e-2, 9))
+    b = N.Batch([n], 1.0, 1.0, precision=precision)
+    b.shear(0, 0.05)
+    r = b.relax(N.relax_params(dt_max=0.2, fcut=1e-10))[0]
+    assert r["status"] == N.CONVERGED
+    pos = b.positions(0)
+    # the device's final image shift equals the oracle's after the same step
+    p0 = o.pos()
+    o.shear(p0, 0.05)
+    info = n.info()
+    area = info.width * (info.height if info.height > 0 else info.maxy - info.miny)
+    w = o.shear_virial(1.0, 1.0, pos)
+    assert abs(r["sxy"] * area - w) <= 1e-10 * abs(w), (r["sxy"] * area, w)

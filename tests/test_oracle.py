@@ -394,3 +394,42 @@ def test_protocol_outputs_formats(tmp_path):
     assert float(cols[3]) >= 0 and abs(float(cols[4]) - float(cols[2]) - float(cols[3])) < 1e-9
     lines = open(base_out + "_1.pos").read().splitlines()
     assert len(lines) == n.n and "\t" in lines[0]
+
+
+@pytest.mark.parametrize("le", [True, False])
+def test_shear_virial_is_the_energy_derivative(le):
+    """The shear virial equals dE/dgamma (central difference of the energy
+    under the affine shear step, image shift included)."""
+    o = O.Net.lattice(14, 0.72, 21, le).triples().fiber_kappa(1e-3, 1e-1, 5)
+    pos = o.pos()
+    rng = np.random.RandomState(3)
+    pos += 0.05 * rng.standard_normal(pos.shape)
+    o.shear(pos, 0.03)
+    w = o.shear_virial(1.0, 1.0, pos)
+    eps = 1e-5
+    p = pos.copy()
+    o.shear(p, eps)
+    ep = sum(o.energies(1.0, 1.0, p))
+    o.shear(p, -2 * eps)
+    em = sum(o.energies(1.0, 1.0, p))
+    o.shear(p, eps)
+    fd = (ep - em) / (2 * eps)
+    assert abs(w - fd) <= 1e-6 * abs(w), (w, fd)
+
+
+def test_shear_virial_full_lattice_closed_form():
+    """Full lattice (p = 1) under affine shear gamma: W = dE/dgamma of
+    E = N ks/2 [(d60 - 1)^2 + (d120 - 1)^2], d^2 = 1 +- s gamma + s^2 gamma^2,
+    s = sqrt(3)/2 (bending terms vanish: fibers stay straight)."""
+    L, g = 16, 0.04
+    o = O.Net.lattice(L, 1.0, 1, True).triples()
+    pos = o.pos()
+    o.shear(pos, g)
+    s = np.sqrt(3.0) / 2
+    d60 = np.sqrt(1 + s * g + (s * g) ** 2)
+    d120 = np.sqrt(1 - s * g + (s * g) ** 2)
+    dd60 = (s + 2 * s * s * g) / (2 * d60)
+    dd120 = (-s + 2 * s * s * g) / (2 * d120)
+    w_exact = L * L * ((d60 - 1) * dd60 + (d120 - 1) * dd120)
+    w = o.shear_virial(1.0, 1e-3, pos)
+    assert abs(w - w_exact) <= 1e-12 * abs(w_exact), (w, w_exact)
