@@ -101,7 +101,7 @@ def main():
                          "L=32 network vs the oracle; ramp: L=512, 100 shear increments, "
                          "FIRE and CG; large: L=2048 mixed vs FP64 per increment")
     ap.add_argument("--increments", type=int, default=0,
-                    help="ramp/large: increments to run (0 = 100 / 2)")
+                    help="ramp/large: increments to run (0 = 20 / 2)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy(args.calibrate_copy)
@@ -282,10 +282,11 @@ def single_leg(N, args):
 
 
 def ramp_leg(N, args):
-    """L=512 (p 0.6, kappa 1e-4) Lees-Edwards network, shear 0 -> 0.5 in 100
-    increments of 0.005, each relaxed to |F|max < 1e-10 from the previous
-    state, FIRE and CG, the whole ramp in one device launch (nr_batch_ramp)."""
-    ninc = args.increments or 100
+    """L=512 (p 0.6, kappa 1e-4) Lees-Edwards network, shear ramp in
+    increments of 0.005 (100 reach 0.5; 20 by default), each relaxed to
+    |F|max < fcut from the previous state, FIRE and CG, ten increments per
+    device launch (nr_batch_ramp)."""
+    ninc = args.increments or 20
     n = N.Network.lattice(512, 0.6, 2, True).triples()
     nodes, edges, tris = n.info().n_nodes, n.info().n_bonds, n.info().n_triples
     incs = np.array([(0.005 * n.info().height, 0.0)] * ninc)
@@ -294,8 +295,8 @@ def ramp_leg(N, args):
            "nodes": nodes, "bonds": edges, "triples": tris}
     for name, method in (("fire", N.METHOD_FIRE), ("cg", N.METHOD_CG)):
         b = N.Batch([n], 1.0, 1e-4)
-        prm = N.relax_params(fcut=1e-10, criterion=N.CRIT_MAX, method=method,
-                             max_steps=1000000, **dict(FIRE, dt_max=0.1))
+        prm = N.relax_params(fcut=args.fcut, criterion=N.CRIT_MAX, method=method,
+                             max_steps=1000000, **FIRE)
         t = time.perf_counter()
         reps, kms = [], 0.0
         for c0 in range(0, ninc, 10):  # ten increments per launch, a progress line each
@@ -323,35 +324,42 @@ def ramp_leg(N, args):
 
 def large_leg(N, args):
     """L=2048 (p 0.62, per-fiber kappa log-U[1e-5,1e-2]) Lees-Edwards network,
-    shear increments of 0.005 from the unstrained state, mixed precision
-    (FP32 stage + FP64 polish) against FP64, per increment: energy and shear
-    stress relative error, time."""
+    shear increments of 0.005 from the unstrained state, FIRE to |F|max < 1e-9,
+    mixed precision (FP32 stage + FP64 polish) against FP64, per increment:
+    energy and shear stress relative error, FIRE steps.  Run through the
+    serving loop (20,000-step launches, bit-identical to one launch) so that
+    the minutes-long relaxes report progress."""
     ninc = args.increments or 2
     n = N.Network.lattice(2048, 0.62, 3, True).triples().fiber_kappa(1e-5, 1e-2, 3)
     progress("large: network built")
     incs = np.array([(0.005 * n.info().height, 0.0)] * ninc)
-    prm = N.relax_params(fcut=1e-10, criterion=N.CRIT_MAX, max_steps=3000000,
-                         **dict(FIRE, dt_max=0.1))
+    prm = N.relax_params(fcut=args.fcut, criterion=N.CRIT_MAX, max_steps=args.max_steps,
+                         **FIRE)
     res = {}
     for name, prec in (("mixed", N.PREC_MIXED), ("fp64", N.PREC_FP64)):
-        b = N.Batch([n], 1.0, 1.0, precision=prec)
-        t = time.perf_counter()
-        reps, kms = [], 0.0
-        for i in range(ninc):
-            reps += b.ramp(incs[i:i + 1], prm)[0]
-            kms += b.last_ms()
-            progress(f"large {name}: increment {i + 1}/{ninc}, {reps[-1]['steps']} steps, "
-                     f"{b.last_ms() / 1e3:.1f} s")
-        res[name] = (reps, kms, time.perf_counter() - t)
-        del b
+        srv = N.Server(precision=prec, slice_steps=20000, hbm_bytes=4 << 30)
+        t0 = time.perf_counter()
+        srv.submit(n, 1.0, 1.0, prm, incs)
+        out = []
+        last = t0
+        while not out:
+            srv.step()
+            out = srv.results()
+            if time.perf_counter() - last > 10.0:
+                last = time.perf_counter()
+                progress(f"large {name}: {srv.stats()['slices']} launches")
+        info = srv.stats()
+        res[name] = (out[0]["reports"], info["busy_ms"], time.perf_counter() - t0)
+        progress(f"large {name}: done in {res[name][2]:.1f} s")
+        del srv
     rm, rf = res["mixed"][0], res["fp64"][0]
     rows = []
-    for i in range(ninc):
+    for i in range(min(len(rm), len(rf))):
         em, ef = rm[i]["es"] + rm[i]["eb"], rf[i]["es"] + rf[i]["eb"]
         rows.append({"gamma": 0.005 * (i + 1), "energy_rel": abs(em - ef) / ef,
                      "stress_rel": abs(rm[i]["sxy"] - rf[i]["sxy"]) / abs(rf[i]["sxy"]),
                      "steps_mixed": rm[i]["steps"], "fp64_polish_steps": rm[i]["fp64_steps"],
-                     "steps_fp64": rf[i]["steps"]})
+                     "steps_fp64": rf[i]["steps"], "status": [rm[i]["status"], rf[i]["status"]]})
     return {"metric": f"config 3: L=2048 mixed vs FP64, {ninc} shear increments of 0.005",
             "unit": "s", "value": res["mixed"][1] * 1e-3, "higher_is_better": False,
             "fp64_s": res["fp64"][1] * 1e-3, "nodes": n.info().n_nodes,
