@@ -94,7 +94,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
-                    help="only run a device copy of MIB MiB (counter calibration)")
+                    help="also run a device copy of MIB MiB after the timed work, as "
+                         "the counter calibration (alone with --steps 0)")
     ap.add_argument("--workload", default="batched",
                     choices=["batched", "single", "ramp", "large"],
                     help="batched: the headline (512 uneven networks); single: one "
@@ -103,8 +104,8 @@ def main():
     ap.add_argument("--increments", type=int, default=0,
                     help="ramp/large: increments to run (0 = 20 / 2)")
     args = ap.parse_args()
-    if args.calibrate_copy:
-        calibrate_copy(args.calibrate_copy)
+    if args.calibrate_copy and args.workload == "batched" and args.steps == 0:
+        print(json.dumps(calibrate_copy(args.calibrate_copy)), flush=True)
         return
     if args.workload != "batched":
         if args.gpus != 1:
@@ -248,6 +249,8 @@ def main():
                      "alg_bytes_per_launch": total_bytes},
         "cpu_baseline": cpu,
     }
+    if args.calibrate_copy:  # after the timed work, for the counter passes
+        out["copy_calibration"] = calibrate_copy(args.calibrate_copy)
     print(json.dumps(out), flush=True)
 
 
@@ -452,16 +455,26 @@ def served_leg(N, args):
 
 
 def calibrate_copy(mib):
-    """A plain device-to-device copy kernel of known size (torch's elementwise
-    copy), so that the FETCH_SIZE / WRITE_SIZE counters can be scaled."""
-    import torch
-    n = mib * (1 << 20) // 4
-    a = torch.ones(n, dtype=torch.float32, device="cuda")
-    b = torch.empty_like(a)
+    """A plain device-to-device copy of known size (the HIP runtime's copy
+    kernel, three launches), so that the FETCH_SIZE / WRITE_SIZE counters of
+    the relax kernel can be scaled by what they read for it."""
+    import ctypes as C
+    path = "/opt/rocm/lib/libamdhip64.so"
+    hip = C.CDLL(path if os.path.exists(path) else "libamdhip64.so")
+    n = C.c_size_t(mib << 20)
+    a, b = C.c_void_p(), C.c_void_p()
+    for rc in (hip.hipMalloc(C.byref(a), n), hip.hipMalloc(C.byref(b), n),
+               hip.hipMemset(a, 1, n)):
+        if rc:
+            raise RuntimeError(f"HIP error {rc} in the copy calibration")
     for _ in range(3):
-        b.copy_(a)
-    torch.cuda.synchronize()
-    print(json.dumps({"calibration_copy_bytes": 4 * n, "launches": 3}), flush=True)
+        if hip.hipMemcpy(b, a, n, 3):  # hipMemcpyDeviceToDevice
+            raise RuntimeError("hipMemcpy failed in the copy calibration")
+    hip.hipDeviceSynchronize()
+    hip.hipFree(a)
+    hip.hipFree(b)
+    progress(f"copy calibration: 3 x {mib} MiB device-to-device")
+    return {"calibration_copy_bytes": mib << 20, "launches": 3}
 
 
 def cpu_baseline(specs, reps, args):

@@ -58,7 +58,7 @@ using nr::hip_fail;
 
 // Default tiles per team member: the best measured on MI355X for networks
 // of 1 to 150 tiles of 256 nodes.
-constexpr int kDefaultTilesPerMember = 4;
+constexpr int kDefaultTilesPerMember = 6;
 // Default largest team: every resident workgroup of an MI355X at two 256-thread
 // workgroups per CU (a fixed number, so that team sizes do not depend on the
 // batch; the resident count still bounds it).
@@ -78,7 +78,7 @@ struct LaunchDev {
 struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
-    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 4;
+    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 6;
     int resident = 0;             // workgroups the relax kernel keeps resident
     int method = NR_METHOD_FIRE;  // of the last relax call (one method per call)
     size_t lds = 0;
