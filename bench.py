@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--plain-sample", type=int, default=8)
     ap.add_argument("--served", type=int, default=64, help="served requests (0 = skip)")
     ap.add_argument("--served-lmax", type=int, default=128)
+    ap.add_argument("--served-tpm", type=int, default=0,
+                    help="tiles per team member of the server (0 = its default)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
@@ -393,7 +395,8 @@ def served_leg(N, args):
     progress(f"served: built {n} requests")
 
     def run(arrivals):
-        srv = N.Server(precision=N.PREC_MIXED, slice_steps=1000)
+        srv = N.Server(precision=N.PREC_MIXED, slice_steps=1000,
+                       tiles_per_member=args.served_tpm)
         t0 = srv.clock()
         i = done = 0
         res = {}

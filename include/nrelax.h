@@ -256,7 +256,8 @@ typedef struct {
 } nr_server_info;
 
 /* shape: precision, block, tile_nodes, team, device, tiles_per_member,
- * mixed_switch as for nr_batch_create (NULL = defaults). */
+ * mixed_switch as for nr_batch_create (NULL = defaults; the server's default
+ * is one tile per team member, the widest teams, for latency). */
 int nr_server_create(const nr_batch_opts *shape, const nr_server_opts *opts,
                      nr_server **out);
 /* Seconds since the server was created (the clock of nr_job_result). */

@@ -133,7 +133,7 @@ constexpr int kMaxClasses = 64;
 
 struct nr_server {
     int device = 0, precision = NR_PREC_FP64, block = 256, npt = 1, tile_nodes = 256;
-    int team_cap = 256, tiles_per_member = 6, slice_steps = 2000, stall_steps = 2000;
+    int team_cap = 256, tiles_per_member = 1, slice_steps = 2000, stall_steps = 2000;
     double switch_ratio = 3e-2;
     int64_t lds_budget = 80 * 1024;
     size_t lds = 0;
@@ -578,7 +578,10 @@ int nr_server_create(const nr_batch_opts *shape, const nr_server_opts *opts, nr_
     s->npt = s->tile_nodes > s->block ? 2 : 1;
     s->lds_budget = nr::lds_per_workgroup(s->block) - nr::kLdsReserve;
     s->lds = (size_t)(nr::kCacheBytes + s->lds_budget);
-    s->tiles_per_member = shape && shape->tiles_per_member > 0 ? shape->tiles_per_member : 6;
+    // a served mix holds tens of jobs, not hundreds: the widest teams (one
+    // tile per member) give the lowest latency and, measured, the highest
+    // throughput there
+    s->tiles_per_member = shape && shape->tiles_per_member > 0 ? shape->tiles_per_member : 1;
     const int64_t hbm = opts && opts->hbm_bytes > 0 ? opts->hbm_bytes : (int64_t)8 << 30;
     s->slot_cap = opts && opts->max_jobs > 0 ? opts->max_jobs : 4096;
     s->slice_steps = opts && opts->slice_steps > 0 ? opts->slice_steps : 2000;
