@@ -197,7 +197,8 @@ void nr_batch_destroy(nr_batch *b);
  * shape is timed by a probe relax of probe_steps FIRE steps per network.
  * opts is read (precision, device, team; a tiles_per_member > 0 is kept) and
  * written with the choice; *predicted_ms (may be NULL) estimates a relax of
- * about 150 sqrt(N) steps per network.  Results depend on the shape only
+ * the expected step counts (a fit over network size and bond density).
+ * Results depend on the shape only
  * through reduction order, so use one choice for every batch whose results
  * are compared. */
 int nr_autotune(int count, nr_network *const *nets, const double *ks,

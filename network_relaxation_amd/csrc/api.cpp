@@ -58,7 +58,7 @@ using nr::hip_fail;
 
 // Default tiles per team member: the best measured on MI355X for networks
 // of 1 to 150 tiles of 256 nodes.
-constexpr int kDefaultTilesPerMember = 6;
+constexpr int kDefaultTilesPerMember = 5;
 // Default largest team: every resident workgroup of an MI355X at two 256-thread
 // workgroups per CU (a fixed number, so that team sizes do not depend on the
 // batch; the resident count still bounds it).
@@ -78,8 +78,10 @@ struct LaunchDev {
 struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
-    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 6;
+    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 5;
     int resident = 0;             // workgroups the relax kernel keeps resident
+    std::vector<double> hist_steps;  // steps of each network in the last relax
+    bool hist_planned = false;       // the current team layout used hist_steps
     int method = NR_METHOD_FIRE;  // of the last relax call (one method per call)
     size_t lds = 0;
     std::vector<std::unique_ptr<LaunchDev>> launches;
@@ -266,7 +268,10 @@ static int set_team_layout(nr_batch *b, int tiles_per_member)
     std::vector<nr::JobSpec> jobs(count);
     for (int k = 0; k < count; k++)
         jobs[k] = {nr::team_size(P.nets[k].n_tiles, b->team_cap, b->tiles_per_member),
-                   P.nets[k].n_tiles, nr::default_steps(P.nets[k].n_nodes)};
+                   P.nets[k].n_tiles,
+                   b->hist_steps.empty()
+                       ? nr::expected_steps(P.nets[k].n_nodes, P.nets[k].n_bonds)
+                       : b->hist_steps[k]};
     std::vector<nr::TeamLayout> layouts = nr::plan_launches(jobs, b->resident);
     b->launches.clear();
     b->team_of_net.assign(count, 1);
@@ -507,19 +512,41 @@ static int set_params(nr_batch *b, const nr_relax_params *params, int shared)
     return NR_OK;
 }
 
+// Step counts of the batch's last relax become the cost estimates of the
+// next one (the class allocation and queue order only: team sizes, and so
+// the arithmetic, do not change).
+static int plan_from_history(nr_batch *b)
+{
+    // a captured graph holds the current tables: keep them while it lives
+    if (b->hist_steps.empty() || b->hist_planned || b->gexec) return NR_OK;
+    b->hist_planned = true;
+    return set_team_layout(b, b->tiles_per_member);
+}
+
+static void record_history(nr_batch *b, const std::vector<double> &steps)
+{
+    b->hist_steps = steps;
+    for (double &v : b->hist_steps) v = std::max(v, 1.0);
+    b->hist_planned = false;
+}
+
 int nr_batch_relax(nr_batch *b, const nr_relax_params *params, int shared, nr_report *reports)
 {
     if (!b || !params) return fail(NR_E_ARG, "null argument");
     int rc = set_params(b, params, shared);
     if (rc) return rc;
+    rc = plan_from_history(b);
+    if (rc) return rc;
     rc = run_kernel(b, nr::kModeRelax);
     if (rc) return rc;
-    if (reports) {
-        std::vector<nr::NetOut> o(b->hnet.size());
-        HIP_TRY(hipMemcpy(o.data(), b->dout.p, sizeof(nr::NetOut) * o.size(),
-                          hipMemcpyDeviceToHost));
-        for (size_t k = 0; k < o.size(); k++) nr::to_report(o[k], reports[k]);
+    std::vector<nr::NetOut> o(b->hnet.size());
+    HIP_TRY(hipMemcpy(o.data(), b->dout.p, sizeof(nr::NetOut) * o.size(), hipMemcpyDeviceToHost));
+    std::vector<double> steps(o.size());
+    for (size_t k = 0; k < o.size(); k++) {
+        steps[k] = (double)o[k].steps;
+        if (reports) nr::to_report(o[k], reports[k]);
     }
+    record_history(b, steps);
     return NR_OK;
 }
 
@@ -548,15 +575,23 @@ int nr_batch_ramp(nr_batch *b, int n_inc, const double *incs, const nr_relax_par
         b->hnet[k].inc_base = 0;
         b->hnet[k].out_base = k * n_inc;
     }
-    rc = run_kernel(b, nr::kModeRelax);
+    rc = plan_from_history(b);
+    if (!rc) rc = run_kernel(b, nr::kModeRelax);
     for (int k = 0; k < count; k++) b->hnet[k].inc_count = 0;
     if (rc) return rc;
-    if (reports) {
-        std::vector<nr::NetOut> o((size_t)count * n_inc);
-        HIP_TRY(hipMemcpy(o.data(), b->inc_out.p, sizeof(nr::NetOut) * o.size(),
-                          hipMemcpyDeviceToHost));
+    std::vector<nr::NetOut> o((size_t)count * n_inc);
+    HIP_TRY(hipMemcpy(o.data(), b->inc_out.p, sizeof(nr::NetOut) * o.size(),
+                      hipMemcpyDeviceToHost));
+    std::vector<double> steps(count, 0.0);
+    for (int k = 0; k < count; k++)
+        for (int i = 0; i < n_inc; i++) {
+            // a ramp that stopped early leaves its later reports untouched
+            const nr::NetOut &r = o[(size_t)k * n_inc + i];
+            if (r.status == NR_CONVERGED || r.status == NR_NOT_CONVERGED) steps[k] += r.steps;
+        }
+    if (reports)
         for (size_t i = 0; i < o.size(); i++) nr::to_report(o[i], reports[i]);
-    }
+    record_history(b, steps);
     return NR_OK;
 }
 
@@ -754,7 +789,7 @@ void nr_batch_destroy(nr_batch *b) { delete b; }
 // the tiles-per-member rule (teams.cpp), then a probe relax of probe_steps
 // FIRE steps per network (no convergence test) measures that shape's cost
 // per model unit on this GPU; the shape with the lowest predicted time for
-// a relax of about 150 sqrt(N) steps per network wins.
+// a relax of the expected step counts (teams.hpp) wins.
 int nr_autotune(int count, nr_network *const *nets, const double *ks, const double *kb,
                 int probe_steps, nr_batch_opts *opts, double *predicted_ms)
 {
@@ -782,7 +817,7 @@ int nr_autotune(int count, nr_network *const *nets, const double *ks, const doub
         std::vector<double> full(count), probe(count, (double)probe_steps);
         for (int k = 0; k < count; k++) {
             tiles[k] = b->plan.nets[k].n_tiles;
-            full[k] = 150.0 * nr::default_steps(b->plan.nets[k].n_nodes);
+            full[k] = nr::expected_steps(b->plan.nets[k].n_nodes, b->plan.nets[k].n_bonds);
         }
         const int tpm = opts->tiles_per_member > 0
                             ? opts->tiles_per_member

@@ -293,6 +293,16 @@ __device__ bool team_sync(const KArgs &A, Team &T, int *s_flag)
 
 // Block + team reduction of a Red4 in a fixed order (wave butterfly, waves
 // in order, members in order).  Every thread gets the result.
+// First triple of thread tid in the element phase: triple chunk c (64
+// consecutive triples) goes to wave (c + bond chunks) mod (B / 64).
+template <int B>
+__device__ inline int tri_first(int tid, int nb)
+{
+    constexpr int W = B / 64;
+    const int wave = tid >> 6, nbc = (nb + 63) >> 6;
+    return ((wave - nbc % W + W) % W) * 64 + (tid & 63);
+}
+
 template <int B>
 __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_red, int *s_flag)
 {
@@ -776,7 +786,9 @@ __device__ Red4 pass32(const KArgs &A, const Team &T, const NetCtx &C, const Ste
             const float coef = ks * d2ml2 / (d * (d + (float)L64));
             L.fv[e] = make_float2(coef * rx, coef * ry);
         }
-        for (int t = tid; t < td.nt; t += B) {
+        // triples are dealt to the waves starting where the bonds' 64-element
+        // chunks ended, so that every wave gets the same number of chunks
+        for (int t = tri_first<B>(tid, td.nb); t < td.nt; t += B) {
             const uint64_t code = bb.tc[t];
             const int h = (int)(code & 0xFFFF), a = (int)((code >> 16) & 0xFFFF),
                       c = (int)((code >> 32) & 0xFFFF);
@@ -1026,7 +1038,9 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 L.fv[e] = make_double2(s * rx, s * ry);
             }
         }
-        for (int t = tid; t < td.nt; t += B) {
+        // triples are dealt to the waves starting where the bonds' 64-element
+        // chunks ended, so that every wave gets the same number of chunks
+        for (int t = tri_first<B>(tid, td.nb); t < td.nt; t += B) {
             int h;
             double ax, ay, bx, by;
             tri_arms(bb, L.pos, C, t, h, ax, ay, bx, by);

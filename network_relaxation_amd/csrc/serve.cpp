@@ -223,9 +223,9 @@ namespace {
 
 double expected_steps(const Job &j, int slice_steps)
 {
-    // steps to converge one increment grow like the lattice side; a launch
-    // gives at most slice_steps
-    const double s = 150.0 * nr::default_steps(j.net.n) * (double)j.incs.size();
+    // expected steps of the job's increments; a launch gives at most
+    // slice_steps
+    const double s = nr::expected_steps(j.net.n, j.plan.nets[0].n_bonds) * (double)j.incs.size();
     return std::min(s, (double)slice_steps);
 }
 

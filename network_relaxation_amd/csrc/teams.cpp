@@ -22,7 +22,7 @@ int team_size(int n_tiles, int cap, int tiles_per_member)
 double job_cost(double steps, int n_tiles, int g)
 {
     const double passes = (double)((n_tiles + g - 1) / g);
-    return steps * (passes + (g > 1 ? 0.25 : 0.0));
+    return steps * (passes + 0.45 * std::log2((double)g));
 }
 
 namespace {

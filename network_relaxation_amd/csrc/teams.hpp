@@ -27,13 +27,23 @@ struct TeamLayout {
 // alone and inside any batch.
 int team_size(int n_tiles, int cap, int tiles_per_member);
 
-// Cost model of one relax, fitted to measured batch runs on MI355X: a FIRE
-// step of a network costs the tile passes of its busiest member plus a team
-// barrier worth a quarter of a tile pass, and the number of steps grows like
-// the lattice side.  `steps` is the expected step count (any unit).
+// Cost model of one relax, fitted to per-network timelines of a 512-network
+// batch on MI355X: a FIRE step of a network costs the tile passes of its
+// busiest member (~4.6 us each) plus the team's wait for its slowest member
+// and the partial-sum exchange, ~0.45 tile passes per doubling of the team.
+// `steps` is the expected step count.
 double job_cost(double steps, int n_tiles, int g);
-// Expected steps of a network with no history: the lattice side.
-inline double default_steps(int n_nodes) { return std::sqrt((double)n_nodes); }
+// Expected FIRE steps of a relax with no history, fitted to 512 relaxed
+// triangular networks (L 16..192, bond fraction p 0.55..0.75, shear 0..0.4):
+// steps ~ L^0.71 exp(-8.2 p), L = sqrt(N), p = mean coordination / 6
+// (R^2 = 0.79 on log steps; L alone explains 0.34).  Sparser networks relax
+// far more slowly, which a size-only estimate misses.
+inline double expected_steps(int n_nodes, int n_bonds)
+{
+    const double n = n_nodes > 0 ? (double)n_nodes : 1.0;
+    const double p = (2.0 * n_bonds) / (6.0 * n);
+    return std::exp(11.70 + 0.707 * std::log(std::sqrt(n)) - 8.24 * p);
+}
 
 // One job of a launch plan: its team size, tile count and expected steps.
 struct JobSpec {

@@ -96,7 +96,8 @@ def test_served_job_equals_one_launch_ramp(precision):
     prm = N.relax_params(**PRM)
     for (n, kb, incs), r in zip(jobs, res):
         assert r["error"] == N.OK and r["n_inc"] == len(incs)
-        b = N.Batch([n], 1.0, kb, precision=precision)
+        # same shape as the server's (one tile per team member): same arithmetic
+        b = N.Batch([n], 1.0, kb, precision=precision, tiles_per_member=1)
         ramp = b.ramp(incs, prm)[0]
         assert [_key(x) for x in r["reports"]] == [_key(x) for x in ramp]
         np.testing.assert_array_equal(r["pos"], b.positions(0))
