@@ -283,11 +283,11 @@ static int set_team_layout(nr_batch *b, int tiles_per_member)
         HIP_TRY(ld->class_jobs.upload(L.class_jobs));
         HIP_TRY(ld->class_off.upload(L.class_off));
         HIP_TRY(ld->queues.alloc((size_t)32 * (L.class_off.size() - 1)));
-        HIP_TRY(ld->mailbox.alloc((size_t)32 * std::max(1, L.n_teams)));
+        HIP_TRY(ld->mailbox.alloc((size_t)32 * std::max(1, L.n_ids)));
         HIP_TRY(ld->partials.alloc((size_t)nr::kPartWords * std::max(1, L.n_parts)));
-        HIP_TRY(ld->bar.alloc((size_t)nr::kBarStride * std::max(1, L.n_teams)));
+        HIP_TRY(ld->bar.alloc((size_t)nr::kBarStride * std::max(1, L.n_ids)));
         HIP_TRY(hipMemset(ld->bar.p, 0, sizeof(unsigned long long) * nr::kBarStride *
-                                             std::max(1, L.n_teams)));
+                                             std::max(1, L.n_ids)));
         ld->L = std::move(L);
         b->launches.push_back(std::move(ld));
     }
@@ -463,6 +463,7 @@ static int run_kernel(nr_batch *b, int mode)
         a.partials = ld->partials.p;
         a.bar = ld->bar.p;
         a.mailbox = ld->mailbox.p;
+        std::memcpy(a.cls_of_log2g, ld->L.cls_of_log2g, sizeof(a.cls_of_log2g));
         const int method = mode == nr::kModeEval ? NR_METHOD_FIRE : b->method;
         int e = nr::launch_relax(a, b->block, b->npt, method, ld->L.grid, b->lds, ld->L.cooperative,
                                  b->stream);
@@ -620,6 +621,7 @@ static int capture_step(nr_batch *b)
         a.partials = ld->partials.p;
         a.bar = ld->bar.p;
         a.mailbox = ld->mailbox.p;
+        std::memcpy(a.cls_of_log2g, ld->L.cls_of_log2g, sizeof(a.cls_of_log2g));
         if (!e) e = nr::launch_relax(a, b->block, b->npt, b->method, ld->L.grid, b->lds, false, st);
     }
     if (!e) e = (int)hipMemcpyAsync(b->h_out, b->dout.p, sizeof(nr::NetOut) * count,

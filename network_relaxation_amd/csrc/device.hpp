@@ -61,9 +61,13 @@ struct BlockRole {
     int member;   // index within the team
     int g;        // team size
     int cls;      // job class (team-size class) the team serves
-    int part;     // first partial-sum slot of the team (4 doubles per member)
-    int pad[3];
+    int part;     // first partial-sum slot of the team (one per member)
+    int tree;     // first barrier/mailbox id of the team's split tree (2g - 1 ids:
+                  // the team, its halves, their halves, ... in heap order)
+    int pad[2];
 };
+
+constexpr int kMaxLog2Team = 16;
 
 struct KArgs {
     double2 *x[2], *v[2], *f[2];
@@ -88,6 +92,7 @@ struct KArgs {
     int *mailbox;             // per team, stride 32
     unsigned *err;            // nonzero after a bounded wait expired
     long long timeout_ticks;
+    int cls_of_log2g[kMaxLog2Team];  // class serving teams of 2^i workgroups, -1 none
     NetRun *run;              // per network resumable state
     const double2 *incs;      // strain increments (xdisp, ydisp)
     NetOut *inc_out;          // one report per increment, at NetDev::out_base

@@ -1509,20 +1509,23 @@ __device__ __forceinline__ void kernel_body(const KArgs &A)
     const BlockRole role = A.roles[blockIdx.x];
     if (role.team < 0) return;  // idle workgroup
     Team T;
-    T.id = role.team;
     T.member = role.member;
     T.g = role.g;
     T.part = role.part;
-    T.gen = 0;
     T.rtag = 0;
+    int heap = 0;  // position in the team's split tree (0: the whole team)
+    T.id = role.tree;
+    int cls = role.cls;
+    // barrier generation of the (sub)team's words (they persist across launches)
+    T.gen = 0;
     if (T.g > 1 && threadIdx.x == 0)
         T.gen = __hip_atomic_load(A.bar + (size_t)T.id * kBarStride + 16, __ATOMIC_RELAXED,
                                   __HIP_MEMORY_SCOPE_AGENT);
-    int *queue = A.queues + 32 * role.cls;
-    const int *jobs = A.class_jobs + A.class_off[role.cls];
-    const int n_jobs = A.class_off[role.cls + 1] - A.class_off[role.cls];
 
     for (;;) {
+        int *queue = A.queues + 32 * cls;
+        const int *jobs = A.class_jobs + A.class_off[cls];
+        const int n_jobs = A.class_off[cls + 1] - A.class_off[cls];
         if (threadIdx.x == 0 && T.member == 0) {
             const int j = __hip_atomic_fetch_add(queue, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             s_job = j;
@@ -1537,9 +1540,34 @@ __device__ __forceinline__ void kernel_body(const KArgs &A)
         __syncthreads();
         const int job = s_job;
         __syncthreads();
-        if (job >= n_jobs) return;
-        relax_network<B, NPT, CG>(A, T, jobs[job], L, s_red, &s_flag);
-        if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+        if (job < n_jobs) {
+            relax_network<B, NPT, CG>(A, T, jobs[job], L, s_red, &s_flag);
+            if (__hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) return;
+            continue;
+        }
+        // This class has no job left (every member saw the same answer): the
+        // team halves, each half serving the class of teams of its size, so
+        // no workgroup idles while smaller classes still queue work.  A job
+        // of a class is still relaxed by a team of that class's size with
+        // members 0..g-1, so its arithmetic does not change.
+        int next = -1;
+        while (T.g > 1 && next < 0) {
+            const int half = T.g >> 1, sub = T.member >= half ? 1 : 0;
+            T.member -= sub * half;
+            T.part += sub * half;
+            T.g = half;
+            heap = 2 * heap + 1 + sub;
+            int lg = 0;
+            while ((1 << lg) < half) lg++;
+            next = lg < kMaxLog2Team ? A.cls_of_log2g[lg] : -1;
+        }
+        if (next < 0) return;
+        cls = next;
+        T.id = role.tree + heap;
+        T.gen = 0;
+        if (T.g > 1 && threadIdx.x == 0)
+            T.gen = __hip_atomic_load(A.bar + (size_t)T.id * kBarStride + 16, __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
     }
 }
 

@@ -123,7 +123,7 @@ struct Tables {
     DevBuf<unsigned long long> bar;
     std::vector<nr::BlockRole> h_roles;  // host copies, alive until the launch ends
     std::vector<int> h_jobs, h_off;
-    int grid = 0, n_teams = 0;
+    int grid = 0, n_teams = 0;  // n_teams: barrier/mailbox ids in use
     bool cooperative = false;
 };
 
@@ -420,9 +420,10 @@ int launch_slice(nr_server *s)
         HIP_TRY(t->class_jobs.alloc(s->slot_cap));
         HIP_TRY(t->class_off.alloc(kMaxClasses + 1));
         HIP_TRY(t->queues.alloc(32 * kMaxClasses));
-        HIP_TRY(t->mailbox.alloc((size_t)32 * R));
+        // split trees: at most 2g - 1 ids per team of g, so 2R ids in all
+        HIP_TRY(t->mailbox.alloc((size_t)32 * 2 * R));
         HIP_TRY(t->partials.alloc((size_t)nr::kPartWords * R));
-        HIP_TRY(t->bar.alloc((size_t)nr::kBarStride * R));
+        HIP_TRY(t->bar.alloc((size_t)nr::kBarStride * 2 * R));
         s->tables.push_back(std::move(t));
     }
     nr::KArgs a;
@@ -461,7 +462,7 @@ int launch_slice(nr_server *s)
         for (size_t i = 0; i < L.class_jobs.size(); i++) t.h_jobs[i] = act[L.class_jobs[i]];
         t.h_off = L.class_off;
         t.grid = L.grid;
-        t.n_teams = L.n_teams;
+        t.n_teams = L.n_ids;
         t.cooperative = L.cooperative;
         HIP_TRY(hipMemcpyAsync(t.roles.p, t.h_roles.data(), sizeof(nr::BlockRole) * t.h_roles.size(),
                                hipMemcpyHostToDevice, s->stream));
@@ -484,6 +485,7 @@ int launch_slice(nr_server *s)
         a.partials = t.partials.p;
         a.bar = t.bar.p;
         a.mailbox = t.mailbox.p;
+        std::memcpy(a.cls_of_log2g, L.cls_of_log2g, sizeof(a.cls_of_log2g));
         int e = nr::launch_relax(a, s->block, s->npt, NR_METHOD_FIRE, t.grid, s->lds,
                                  t.cooperative, s->stream);
         if (e) return hip_fail((hipError_t)e, "relax kernel launch");

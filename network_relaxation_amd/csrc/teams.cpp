@@ -91,12 +91,18 @@ TeamLayout plan_teams(const std::vector<JobSpec> &jobs, int resident)
     }
     T.makespan = 0.0;
     for (int i = 0; i < nc; i++) T.makespan = std::max(T.makespan, finish[i]);
-    // roles, job lists
-    int team_id = 0, part = 0;
+    // roles, job lists; a team whose class runs out of jobs halves and serves
+    // the class of half its size (kernel), so each team owns the ids of its
+    // split tree
+    int team_id = 0, part = 0, ids = 0;
+    for (int i = 0; i < kMaxLog2Team; i++) T.cls_of_log2g[i] = -1;
     T.class_off.push_back(0);
     for (int i = 0; i < nc; i++) {
         T.class_jobs.insert(T.class_jobs.end(), members[i].begin(), members[i].end());
         T.class_off.push_back((int)T.class_jobs.size());
+        int lg = 0;
+        while ((1 << lg) < gs[i]) lg++;
+        if (lg < kMaxLog2Team) T.cls_of_log2g[lg] = i;
         for (int t = 0; t < teams[i]; t++) {
             for (int m = 0; m < gs[i]; m++) {
                 BlockRole r = {};
@@ -105,13 +111,16 @@ TeamLayout plan_teams(const std::vector<JobSpec> &jobs, int resident)
                 r.g = gs[i];
                 r.cls = i;
                 r.part = part;
+                r.tree = ids;
                 T.roles.push_back(r);
             }
             team_id++;
             part += gs[i];
+            ids += 2 * gs[i] - 1;
             if (gs[i] > 1) T.cooperative = true;
         }
     }
+    T.n_ids = ids;
     T.grid = (int)T.roles.size();
     T.n_teams = team_id;
     T.n_parts = part;

@@ -13,6 +13,8 @@ struct TeamLayout {
     int grid = 0;                   // workgroups launched
     int n_teams = 0;
     int n_parts = 0;                // partial-sum slots (sum of team sizes)
+    int n_ids = 0;                  // barrier/mailbox ids (split trees, 2g - 1 per team)
+    int cls_of_log2g[kMaxLog2Team]; // class of teams of 2^i workgroups, -1 none
     bool cooperative = false;       // some team has more than one member
     double makespan = 0.0;          // predicted, in cost-model units
     std::vector<BlockRole> roles;   // per workgroup
