@@ -56,9 +56,12 @@ using nr::DevBuf;
 using nr::fail;
 using nr::hip_fail;
 
-// Default tiles per team member: the best measured on MI355X for networks
-// of 1 to 150 tiles of 256 nodes.
-constexpr int kDefaultTilesPerMember = 5;
+// Default tiles per team member: the best measured on MI355X for a batch of
+// 512 networks of 1 to 150 tiles of 256 nodes relaxed again (planned from
+// the step counts of its last relax): 5 -> 79, 7 -> 81, 8 -> 84, 10 -> 80,
+// 12 -> 65 networks/s.  A first relax, planned from estimated step counts,
+// takes 6.5 s at 5 and 6.8 s at 8 (one badly estimated network ends last).
+constexpr int kDefaultTilesPerMember = 8;
 // Default largest team: every resident workgroup of an MI355X at two 256-thread
 // workgroups per CU (a fixed number, so that team sizes do not depend on the
 // batch; the resident count still bounds it).
@@ -78,7 +81,7 @@ struct LaunchDev {
 struct nr_batch {
     int device = 0;
     int precision = NR_PREC_FP64;
-    int npt = 1, block = 256, team_cap = 256, tiles_per_member = 5;
+    int npt = 1, block = 256, team_cap = 256, tiles_per_member = kDefaultTilesPerMember;
     int resident = 0;             // workgroups the relax kernel keeps resident
     std::vector<double> hist_steps;  // steps of each network in the last relax
     bool hist_planned = false;       // the current team layout used hist_steps

@@ -99,13 +99,15 @@ struct KArgs {
     int slice_steps;          // FIRE steps per network per launch, 0 = no limit
 };
 
-// LDS cache, per workgroup, of the tile descriptors and halo node ids of the
-// first kTileCache tiles a team member relaxes in its current network (halo
-// lists up to kCacheHalo ids), placed before the tile area: with the halo ids
+// LDS cache, per workgroup, of the tile descriptors of the first kTileCache
+// tiles a team member relaxes in its current network and of their halo node
+// ids (a pool of kCachePool ids filled in tile order; a tile whose list no
+// longer fits is not cached), placed before the tile area: with the halo ids
 // at hand a tile pass issues its blob, own-node and halo-node loads at once.
-constexpr int kTileCache = 6;
-constexpr int kCacheHalo = 192;
-constexpr int kCacheBytes = kTileCache * (int)sizeof(TileDev) + kTileCache * kCacheHalo * 4;
+// Sized for 8-16 tiles of 256 nodes (~125 halo ids each) per member.
+constexpr int kTileCache = 16;
+constexpr int kCachePool = 1792;
+constexpr int kCacheBytes = kTileCache * (int)sizeof(TileDev) + kTileCache * 4 + kCachePool * 4;
 static_assert(kCacheBytes % 16 == 0, "tile area alignment");
 // LDS of a workgroup outside the tile area: the cache and static scratch.
 constexpr int kLdsReserve = kCacheBytes + 1024;

@@ -394,7 +394,8 @@ struct Lds {
     double2 *pos;
     double2 *fv;
     TileDev *ctd;  // cached tile descriptors of this member
-    int *chalo;    // their halo ids, kCacheHalo per tile
+    int *coff;     // offset of each one's halo ids in chalo, -1 = not cached
+    int *chalo;    // halo id pool
 };
 
 // Descriptor of the kt-th tile of this member: from the LDS cache (every
@@ -711,54 +712,123 @@ __device__ Red4 pass32(const KArgs &A, const Team &T, const NetCtx &C, const Ste
     const bool sh = T.g > 1;
     const float2 zf = make_float2(0.0f, 0.0f);
     const float ks = (float)C.ks;
-    for (int ti = T.member; ti < C.n_tiles; ti += T.g) {
-        const TileDev td = A.tiles[C.tile_base + ti];
+    for (int ti = T.member, kt = 0; ti < C.n_tiles; ti += T.g, kt++) {
+        const bool cached = kt < kTileCache;
+        const TileDev td = tile_desc(A, L0, C.tile_base + ti, kt);
+        const int hoff = cached ? __builtin_amdgcn_readfirstlane(L0.coff[kt]) : -1;
         const int local = td.n_own + td.n_halo;
         Lds32 L;
         L.blob = L0.blob;
         L.anc = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
         L.u = reinterpret_cast<float2 *>(L.anc + local);
         L.fv = L.u + local;
+        // halo ids already in LDS (resident blob or the member's cache): the
+        // blob, own-node and halo-node loads then go out together
+        const int *hid = resident ? reinterpret_cast<const int *>(L0.blob)
+                         : cached && hoff >= 0 ? L0.chalo + hoff
+                                               : nullptr;
+        const uint4 *__restrict__ bsrc = A.blob + td.blob16;
+        uint4 *bdst = reinterpret_cast<uint4 *>(L.blob);
+        const int n16 = resident ? 0 : td.blob_bytes >> 4;
+        uint4 r0, r1, r2, r3, r4, r5, r6, r7;
         if (!resident) {
-            stage_blob<B>(A, td, L.blob);
-            __syncthreads();
+            const int last = n16 - 1;
+            r0 = bsrc[min(tid, last)];
+            r1 = bsrc[min(tid + B, last)];
+            r2 = bsrc[min(tid + 2 * B, last)];
+            r3 = bsrc[min(tid + 3 * B, last)];
+            r4 = bsrc[min(tid + 4 * B, last)];
+            r5 = bsrc[min(tid + 5 * B, last)];
+            r6 = bsrc[min(tid + 6 * B, last)];
+            r7 = bsrc[min(tid + 7 * B, last)];
         }
-        const Blob bb = blob_view(L.blob, td);
-        float2 vh = zf;
-        unsigned char mk = 0;
+        // own node: anchor (or, on the init pass, the position before the
+        // strain step) and FP32 state
         const int k = tid;
+        double2 xo = make_double2(0.0, 0.0);
+        float2 uo = zf, vo = zf, fo = zf;
+        unsigned char mk = 0;
+        if (k < td.n_own) {
+            const int gid = td.node0 + k;
+            xo = ld_node(sh, KIND == kPass32Init ? A.x[p] : A.x[pa], C, gid);
+            if (KIND != kPass32Init) {
+                uo = ld_state32(sh, A.u32[s] + gid);
+                vo = ld_state32(sh, A.v32[s] + gid);
+                fo = ld_state32(sh, A.f32[s] + gid);
+            }
+            mk = A.mask[gid];
+        }
+        // first halo node of this thread (the rest, if any, below)
+        const bool hfirst = hid != nullptr && tid < td.n_halo;
+        double2 hx = make_double2(0.0, 0.0);
+        float2 hu = zf, hv = zf, hf = zf;
+        if (hfirst) {
+            const int gid = hid[tid];
+            hx = ld_node(sh, KIND == kPass32Init ? A.x[p] : A.x[pa], C, gid);
+            if (KIND != kPass32Init) {
+                hu = ld_state32(sh, A.u32[s] + gid);
+                hv = ld_state32(sh, A.v32[s] + gid);
+                hf = ld_state32(sh, A.f32[s] + gid);
+            }
+        }
+        if (!resident) {
+            if (tid < n16) bdst[tid] = r0;
+            if (tid + B < n16) bdst[tid + B] = r1;
+            if (tid + 2 * B < n16) bdst[tid + 2 * B] = r2;
+            if (tid + 3 * B < n16) bdst[tid + 3 * B] = r3;
+            if (tid + 4 * B < n16) bdst[tid + 4 * B] = r4;
+            if (tid + 5 * B < n16) bdst[tid + 5 * B] = r5;
+            if (tid + 6 * B < n16) bdst[tid + 6 * B] = r6;
+            if (tid + 7 * B < n16) bdst[tid + 7 * B] = r7;
+            for (int i = kBlobRound * B + tid; i < n16; i += B) bdst[i] = bsrc[i];
+        }
+        float2 vh = zf;
         if (k < td.n_own) {
             const int gid = td.node0 + k;
             double2 a;
             float2 un;
             if (KIND == kPass32Init) {
                 double2 dummy;
-                a = advance<kPassInit>(ld_state(sh, A.x[p] + gid), make_double2(0.0, 0.0),
-                                       make_double2(0.0, 0.0), S, C, dummy);
+                a = advance<kPassInit>(xo, make_double2(0.0, 0.0), make_double2(0.0, 0.0), S, C,
+                                       dummy);
                 st_state(sh, A.x[pa] + gid, a);
                 un = zf;
             } else {
-                a = ld_state(sh, A.x[pa] + gid);
-                un = advance32<kPass32Step>(ld_state32(sh, A.u32[s] + gid),
-                                            ld_state32(sh, A.v32[s] + gid),
-                                            ld_state32(sh, A.f32[s] + gid), S, vh);
+                a = xo;
+                un = advance32<kPass32Step>(uo, vo, fo, S, vh);
             }
             st_state32(sh, A.u32[s1] + gid, un);
-            mk = A.mask[gid];
             L.anc[k] = a;
             L.u[k] = un;
         }
-        for (int kk = td.n_own + tid; kk < local; kk += B) {
-            const int gid = bb.halo[kk - td.n_own];
+        if (hfirst) {
             double2 a;
             float2 un, dummy;
             if (KIND == kPass32Init) {
                 double2 d2;
-                a = advance<kPassInit>(ld_state(sh, A.x[p] + gid), make_double2(0.0, 0.0),
+                a = advance<kPassInit>(hx, make_double2(0.0, 0.0), make_double2(0.0, 0.0), S, C,
+                                       d2);
+                un = zf;
+            } else {
+                a = hx;
+                un = advance32<kPass32Step>(hu, hv, hf, S, dummy);
+            }
+            L.anc[td.n_own + tid] = a;
+            L.u[td.n_own + tid] = un;
+        }
+        if (hid == nullptr) __syncthreads();  // halo ids come from the blob
+        const Blob bb = blob_view(L.blob, td);
+        for (int kk = td.n_own + (hid != nullptr ? B : 0) + tid; kk < local; kk += B) {
+            const int gid = hid != nullptr ? hid[kk - td.n_own] : bb.halo[kk - td.n_own];
+            double2 a;
+            float2 un, dummy;
+            if (KIND == kPass32Init) {
+                double2 d2;
+                a = advance<kPassInit>(ld_node(sh, A.x[p], C, gid), make_double2(0.0, 0.0),
                                        make_double2(0.0, 0.0), S, C, d2);
                 un = zf;
             } else {
-                a = ld_state(sh, A.x[pa] + gid);
+                a = ld_node(sh, A.x[pa], C, gid);
                 un = advance32<kPass32Step>(ld_state32(sh, A.u32[s] + gid),
                                             ld_state32(sh, A.v32[s] + gid),
                                             ld_state32(sh, A.f32[s] + gid), S, dummy);
@@ -883,6 +953,7 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     for (int ti = T.member, kt = 0; ti < C.n_tiles; ti += T.g, kt++) {
         const bool cached = kt < kTileCache;
         const TileDev td = tile_desc(A, L0, C.tile_base + ti, kt);
+        const int hoff = cached ? __builtin_amdgcn_readfirstlane(L0.coff[kt]) : -1;
         // positions and force vectors follow this tile's blob
         Lds L = L0;
         L.pos = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
@@ -890,8 +961,8 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
         // halo ids already in LDS (resident blob or the member's cache): the
         // blob, own-node and halo-node loads then go out together
         const int *hid = resident ? reinterpret_cast<const int *>(L0.blob)
-                         : (cached && td.n_halo <= kCacheHalo) ? L0.chalo + kt * kCacheHalo
-                                                               : nullptr;
+                         : cached && hoff >= 0 ? L0.chalo + hoff
+                                               : nullptr;
         // The blob's first kBlobRound x 16 B per thread and the own nodes'
         // state are loaded together; the LDS writes then wait only for the
         // loads they use.
@@ -1144,17 +1215,21 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     // a member owning exactly one tile keeps that tile's blob in LDS
     const bool resident = T.member < C.n_tiles && C.n_tiles <= T.member + T.g;
     if (resident) stage_blob<B>(A, A.tiles[C.tile_base + T.member], L.blob);
-    // descriptors and halo ids of the member's first tiles (halo ids lead the blob)
-    for (int k = 0, ti = T.member; k < kTileCache && ti < C.n_tiles; k++, ti += T.g) {
+    // descriptors and halo ids of the member's first tiles (halo ids lead the
+    // blob), the ids packed into the pool while they fit
+    for (int k = 0, ti = T.member, used = 0; k < kTileCache && ti < C.n_tiles; k++, ti += T.g) {
         const TileDev *tg = A.tiles + C.tile_base + ti;
         constexpr int kWords = (int)(sizeof(TileDev) / sizeof(int));
         if (threadIdx.x < kWords)
             reinterpret_cast<int *>(L.ctd + k)[threadIdx.x] =
                 reinterpret_cast<const int *>(tg)[threadIdx.x];
         const int n_halo = tg->n_halo;
-        if (n_halo <= kCacheHalo) {
+        const bool fits = used + n_halo <= kCachePool;
+        if (threadIdx.x == 0) L.coff[k] = fits ? used : -1;
+        if (fits) {
             const int *src = reinterpret_cast<const int *>(A.blob + tg->blob16);
-            for (int i = threadIdx.x; i < n_halo; i += B) L.chalo[k * kCacheHalo + i] = src[i];
+            for (int i = threadIdx.x; i < n_halo; i += B) L.chalo[used + i] = src[i];
+            used += n_halo;
         }
     }
     __syncthreads();
@@ -1501,7 +1576,8 @@ __device__ __forceinline__ void kernel_body(const KArgs &A)
     __shared__ int s_job;
     Lds L;
     L.ctd = reinterpret_cast<TileDev *>(smem);
-    L.chalo = reinterpret_cast<int *>(smem + kTileCache * sizeof(TileDev));
+    L.coff = reinterpret_cast<int *>(smem + kTileCache * sizeof(TileDev));
+    L.chalo = L.coff + kTileCache;
     L.blob = smem + kCacheBytes;
     L.pos = nullptr;  // placed per tile, after its blob
     L.fv = nullptr;
