@@ -16,7 +16,8 @@ per-network energy difference of the mixed form from it), the plain form (one
 network per launch, FP64) on a sample, a served stream (Poisson arrivals of
 strain-ramp jobs through the serving loop, p50/p99 latency), the relax
 kernel's algorithmic HBM rate against MI355X's 8 TB/s, and the CPU FP64 oracle
-on the host cores over a bounded sample.  Prints one JSON line.
+on the host cores over a bounded sample (FIRE, and beside it the reference's
+small-strain linear solve).  Prints one JSON line.
 """
 import argparse
 import json
@@ -525,11 +526,36 @@ def cpu_baseline(specs, reps, args):
     ns = sum(r[0] for r in res)
     rate = ns / wall  # node-steps/s over all threads
     node_steps_per_net = np.mean([reps[k]["steps"] * specs[k]["L"] ** 2 for k in range(len(specs))])
+    progress(f"cpu baseline: FIRE sample done ({wall:.1f} s)")
     return {"value": rate / node_steps_per_net, "unit": "networks/s", "cores": cores,
             "kind": "port",
             "sample": f"{len(idx)} sampled networks x {per_net_steps} FIRE steps on {cores} "
                       f"threads ({wall:.1f} s): {rate:.3e} node-steps/s, divided by the batch's "
-                      f"measured {node_steps_per_net:.3e} node-steps per relaxed network"}
+                      f"measured {node_steps_per_net:.3e} node-steps per relaxed network",
+            "linear": cpu_linear(specs, idx[:4], cores)}
+
+
+def cpu_linear(specs, idx, cores):
+    """The reference's other CPU path, the small-strain linear solve
+    (sparse_relax.cpp), restated in the oracle: stiffness assembly and a CG
+    solve of the reduced system at gamma = 0.01, one network at a time on all
+    threads.  A different problem from FIRE at finite strain (linear response),
+    reported beside it, not as the baseline value."""
+    import oracle as O
+    out = []
+    t0 = time.perf_counter()
+    for k in idx:
+        s = specs[k]
+        o = O.Net.lattice(s["L"], s["p"], s["seed"], True).triples()
+        o.fiber_kappa(1e-5, 1e-2, s["kseed"])
+        t = time.perf_counter()
+        r = o.linear_response(1.0, 1.0, 0.01, max_iter=20000)
+        out.append(dict(L=s["L"], seconds=time.perf_counter() - t, cg_iters=r["iters"],
+                        residual=r["residual"]))
+    wall = time.perf_counter() - t0
+    progress(f"cpu baseline: linear sample done ({wall:.1f} s)")
+    return {"solves_per_s": len(out) / wall, "threads": cores, "gamma": 0.01,
+            "cg_iter_cap": 20000, "networks": out}
 
 
 if __name__ == "__main__":

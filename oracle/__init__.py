@@ -2,7 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by
 bench.py's CPU-baseline leg, as the checker.  The product never imports it.
-The C sources (oracle.c / oracle.h) cite the reference lines they restate.
+The C sources (oracle.c, linear.c, oracle.h) cite the reference lines they
+restate.
 """
 import ctypes as C
 import os
@@ -44,6 +45,11 @@ class OrBatch(C.Structure):
         ("kappa_lo", C.c_double), ("kappa_hi", C.c_double),
         ("kappa_seed", C.c_uint64),
     ]
+
+
+class OrCsr(C.Structure):
+    _fields_ = [("n", C.c_int), ("nnz", C.c_long), ("rowptr", C.POINTER(C.c_int)),
+                ("col", C.POINTER(C.c_int)), ("val", C.POINTER(C.c_double))]
 
 
 _lib = None
@@ -88,6 +94,16 @@ def lib():
         L.or_lattice.argtypes = [C.c_int, C.c_double, C.c_uint64, C.c_int, P(OrNet)]
         L.or_uniform.argtypes = [C.c_uint64, C.c_uint64]
         L.or_uniform.restype = C.c_double
+        L.or_linear_stiffness.argtypes = [P(OrNet), C.c_double, C.c_double, C.c_double,
+                                          P(OrCsr)]
+        L.or_csr_free.argtypes = [P(OrCsr)]
+        L.or_csr_free.restype = None
+        L.or_linear_response.argtypes = [P(OrNet), C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_double, C.c_int, C.c_double,
+                                         C.c_int, dp, dp]
+        L.or_linear_energies.argtypes = [P(OrNet), C.c_double, C.c_double, C.c_double, dp,
+                                         C.c_double, C.c_double, C.c_int, dp, dp]
+        L.or_linear_energies.restype = None
         _lib = L
     return _lib
 
@@ -246,6 +262,41 @@ class Net:
         rc = lib().or_fire(C.byref(self.s), ks, kb, _dp(pos), fpa, nmin, mass, fcut,
                            max_steps, criterion, rep, C.byref(steps))
         return dict(rc=rc, rms=rep[0], es=rep[1], eb=rep[2], fmax=rep[3], steps=steps.value)
+
+    def linear_stiffness(self, ks, kb, mu=0.0):
+        """Assembled stiffness (sparse_relax.cpp:286-428) as a dense 2n x 2n
+        array (small networks only)."""
+        K = OrCsr()
+        if lib().or_linear_stiffness(C.byref(self.s), ks, kb, mu, C.byref(K)):
+            raise MemoryError("or_linear_stiffness failed")
+        try:
+            n = K.n
+            rp = np.ctypeslib.as_array(K.rowptr, shape=(n + 1,))
+            col = np.ctypeslib.as_array(K.col, shape=(max(K.nnz, 1),))[:K.nnz]
+            val = np.ctypeslib.as_array(K.val, shape=(max(K.nnz, 1),))[:K.nnz]
+            D = np.zeros((n, n))
+            for r in range(n):
+                D[r, col[rp[r]:rp[r + 1]]] = val[rp[r]:rp[r + 1]]
+        finally:
+            lib().or_csr_free(C.byref(K))
+        return D
+
+    def linear_response(self, ks, kb, s, c=0.0, mu=0.0, affine=True, tol=1e-12,
+                        max_iter=1000000):
+        """Small-strain response (sparse_relax.cpp:676-881): displacements u,
+        linearised energies, the reduced system's residual, CG iterations."""
+        u = np.zeros(2 * self.s.n)
+        rep = np.zeros(4)
+        rc = lib().or_linear_response(C.byref(self.s), ks, kb, mu, s, c, int(affine), tol,
+                                      max_iter, _dp(u), _dp(rep))
+        return dict(rc=rc, u=u, es=rep[0], eb=rep[1], residual=rep[2], iters=int(rep[3]))
+
+    def linear_energies(self, ks, kb, u, s, c=0.0, mu=0.0, affine=True):
+        u = np.ascontiguousarray(u, dtype=np.float64)
+        es, eb = C.c_double(), C.c_double()
+        lib().or_linear_energies(C.byref(self.s), ks, kb, mu, _dp(u), s, c, int(affine),
+                                 C.byref(es), C.byref(eb))
+        return es.value, eb.value
 
     def cg(self, ks, kb, pos, fcut=1e-10, max_iters=1000000, criterion=CRIT_MAX):
         rep = (C.c_double * 4)()

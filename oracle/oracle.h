@@ -14,6 +14,9 @@
  *   run_fire_md   :596-762  do_relaxation_run :792-1010
  *   parse_batch_file :297-383, report_name :203-213, report_displaced :216-227
  *   get_file_lists (src/file_lists.cpp:11-137)
+ * and, in linear.c, the small-strain solver of src/sparse_relax.cpp
+ * (stiffness assembly :286-428, reduced system :560-605, solve and
+ * energies :49-96, :676-881).
  * plus the extensions the product also carries (Lees-Edwards box, max-force
  * criterion, per-fiber bending stiffness, nonlinear CG).
  *
@@ -115,6 +118,26 @@ int or_run_protocol(const char *net_file, const char *log_name,
 int or_dec_digits(int value);
 void or_report_name(const char *base, int num_digits, int count,
                     const char *ext, char *out, int cap);
+
+/* Linear (small-strain) response, src/sparse_relax.cpp (oracle/linear.c).
+ * K: the assembled stiffness, both triangles, CSR over the 2n DOFs. */
+typedef struct {
+    int n;        /* rows (2 x nodes) */
+    long nnz;
+    int *rowptr;  /* n + 1 */
+    int *col;
+    double *val;
+} or_csr;
+int or_linear_stiffness(const or_net *net, double ks, double kb, double mu, or_csr *K);
+void or_csr_free(or_csr *K);
+/* s, c: shear and compression strains of the affine field (affine != 0), or
+ * the top row's displacements (affine == 0, reference mode only).
+ * u: 2n displacements.  rep: [0] E_stretch+shear, [1] E_bend, [2] residual
+ * 2-norm of the reduced system, [3] CG iterations. */
+int or_linear_response(const or_net *net, double ks, double kb, double mu, double s, double c,
+                       int affine, double tol, int max_iter, double *u, double rep[4]);
+void or_linear_energies(const or_net *net, double ks, double kb, double mu, const double *u,
+                        double s, double c, int affine, double *e_ss, double *e_bend);
 
 /* Synthetic triangular lattice (spacing 1, odd rows shifted by 1/2), each
  * candidate bond (node*3 + direction) kept when u < p. */
