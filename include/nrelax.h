@@ -183,6 +183,27 @@ int nr_batch_set_positions(nr_batch *b, int index, const double *pos);
 /* One force / energy evaluation at the current positions (pending strain
  * applied first), for checking: force[2N], energies[2] = {stretch, bend}. */
 int nr_batch_eval(nr_batch *b, int index, double *force, double *energies);
+/* Post-relax reductions of network `index` at its current state, computed on
+ * the device from the displacements u = x - x0 since the batch was created
+ * (or reset):
+ *   nonaffinity      mean over nodes of |u - u_aff|^2 with u_aff =
+ *                    ((y0 - miny) shear_strain, (y0 - miny) axial_strain):
+ *                    calc_na_val, na_calc.cpp:308-326
+ *   axial_energy     k_axial/2 sum over bonds of (u_ij . rhat_ij)^2, rhat from
+ *                    the input geometry: axial_contribution.cpp:173-190
+ *   mean_bond_strain, max_bond_strain (max |.|) of the bond strains
+ *                    (l - l0) / l0: get_strains, strain_calc.cpp:96-120
+ * bond_strains (NULL or one double per bond) receives the strains in the
+ * network's edge order.  In a Lees-Edwards box the image of a bond's second
+ * node across the y boundary moves with the box shift. */
+typedef struct {
+    double nonaffinity;
+    double axial_energy;
+    double mean_bond_strain;
+    double max_bond_strain;
+} nr_analysis;
+int nr_batch_analyze(nr_batch *b, int index, double shear_strain, double axial_strain,
+                     double k_axial, nr_analysis *out, double *bond_strains);
 /* Device time of the last relax launch in milliseconds. */
 double nr_batch_last_ms(const nr_batch *b);
 /* Bytes one FIRE step moves per the algorithmic count for this batch. */

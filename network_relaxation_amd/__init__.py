@@ -74,6 +74,11 @@ class ServerInfo(C.Structure):
                 ("active", C.c_int), ("waiting", C.c_int), ("done", C.c_int)]
 
 
+class Analysis(C.Structure):
+    _fields_ = [("nonaffinity", C.c_double), ("axial_energy", C.c_double),
+                ("mean_bond_strain", C.c_double), ("max_bond_strain", C.c_double)]
+
+
 class NRError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"nrelax error {code}: {msg}")
@@ -112,6 +117,8 @@ def lib():
         L.nr_batch_get_positions.argtypes = [vp, C.c_int, dp]
         L.nr_batch_set_positions.argtypes = [vp, C.c_int, dp]
         L.nr_batch_eval.argtypes = [vp, C.c_int, dp, dp]
+        L.nr_batch_analyze.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                       P(Analysis), dp]
         L.nr_batch_last_ms.argtypes = [vp]
         L.nr_batch_last_ms.restype = C.c_double
         L.nr_batch_counts.argtypes = [vp, C.c_int, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
@@ -313,6 +320,21 @@ class Batch:
         e = np.zeros(2)
         _check(lib().nr_batch_eval(self.h, index, _dp(f), _dp(e)))
         return f, e
+
+    def analyze(self, index, shear, axial=0.0, k_axial=1.0, strains=False):
+        """Post-relax reductions of network `index` on the device
+        (nr_batch_analyze): non-affinity against the affine field of `shear`
+        and `axial`, axial energy, mean / max |bond strain|, and with
+        strains=True the bond strains in edge order."""
+        a = Analysis()
+        st = np.zeros(max(self.networks[index].info().n_bonds, 1)) if strains else None
+        _check(lib().nr_batch_analyze(self.h, index, shear, axial, k_axial, C.byref(a),
+                                      _dp(st) if strains else None))
+        out = dict(nonaffinity=a.nonaffinity, axial_energy=a.axial_energy,
+                   mean_strain=a.mean_bond_strain, max_strain=a.max_bond_strain)
+        if strains:
+            out["strains"] = st[:self.networks[index].info().n_bonds]
+        return out
 
     def last_ms(self):
         return lib().nr_batch_last_ms(self.h)

@@ -106,6 +106,9 @@ struct nr_batch {
     DevBuf<double2> incs;          // strain ramp increments
     DevBuf<nr::NetOut> inc_out;    // per-increment reports of a ramp
     DevBuf<unsigned> err;
+    // post-relax reductions: per-network bond lists (device node indices),
+    // built on first use
+    std::vector<std::unique_ptr<DevBuf<int4>>> an_bonds;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     double last_ms = 0.0;
@@ -748,6 +751,57 @@ int nr_batch_set_positions(nr_batch *b, int index, const double *pos)
     const int par = b->hnet[index].parity;
     HIP_TRY(hipMemcpy(b->x[par].p + pn.node_base, tmp.data(), sizeof(double2) * pn.n_nodes,
                       hipMemcpyHostToDevice));
+    return NR_OK;
+}
+
+int nr_batch_analyze(nr_batch *b, int index, double shear_strain, double axial_strain,
+                     double k_axial, nr_analysis *out, double *bond_strains)
+{
+    if (!b || !out || index < 0 || index >= (int)b->hnet.size())
+        return fail(NR_E_ARG, "invalid arguments");
+    HIP_TRY(hipSetDevice(b->device));
+    const nr::Network &net = *b->nets[index];
+    const nr::PlanNet &pn = b->plan.nets[index];
+    const nr::NetDev &nd = b->hnet[index];
+    const int ne = net.e();
+    if (b->an_bonds.size() != b->hnet.size()) b->an_bonds.resize(b->hnet.size());
+    if (!b->an_bonds[index] && ne > 0) {
+        std::vector<int4> h(ne);
+        for (int e = 0; e < ne; e++)
+            h[e] = make_int4(pn.iperm[net.bonds[2 * e]], pn.iperm[net.bonds[2 * e + 1]],
+                             net.bwrap[2 * e], net.bwrap[2 * e + 1]);
+        auto buf = std::make_unique<DevBuf<int4>>();
+        HIP_TRY(buf->upload(h));
+        b->an_bonds[index] = std::move(buf);
+    }
+    DevBuf<double> strain, part;
+    HIP_TRY(strain.alloc(std::max(ne, 1)));
+    HIP_TRY(part.alloc(4 * nr::kAnalysisBlocks));
+    const double2 *x = b->x[nd.parity].p + pn.node_base;
+    const double2 *x0 = b->x_init.p + pn.node_base;
+    HIP_TRY((hipError_t)nr::launch_analysis(
+        x, x0, pn.n_nodes, net.miny, shear_strain, axial_strain,
+        ne > 0 ? b->an_bonds[index]->p : nullptr, ne, net.width, net.height, nd.le_shift,
+        strain.p, part.p, b->stream));
+    std::vector<double> hp(4 * nr::kAnalysisBlocks);
+    HIP_TRY(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost,
+                           b->stream));
+    if (bond_strains && ne > 0)
+        HIP_TRY(hipMemcpyAsync(bond_strains, strain.p, sizeof(double) * ne,
+                               hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    // workgroup partials added in workgroup order
+    double na = 0.0, ax = 0.0, sum = 0.0, mx = 0.0;
+    for (int w = 0; w < nr::kAnalysisBlocks; w++) {
+        na += hp[w];
+        ax += hp[nr::kAnalysisBlocks + 3 * w];
+        sum += hp[nr::kAnalysisBlocks + 3 * w + 1];
+        mx = std::max(mx, hp[nr::kAnalysisBlocks + 3 * w + 2]);
+    }
+    out->nonaffinity = pn.n_nodes > 0 ? na / pn.n_nodes : 0.0;
+    out->axial_energy = k_axial * ax;
+    out->mean_bond_strain = ne > 0 ? sum / ne : 0.0;
+    out->max_bond_strain = mx;
     return NR_OK;
 }
 

@@ -141,4 +141,14 @@ int launch_strain_step(NetDev *nets, int count, const double2 *d, void *stream);
 // Zeroes `bytes` (a multiple of 8) with a kernel (graph-capturable).
 int launch_zero(void *p, size_t bytes, void *stream);
 
+// Post-relax reductions of one network (analysis.hip): per-workgroup partial
+// sums over kAnalysisBlocks workgroups into part[0 .. kAnalysisBlocks)
+// (non-affinity) and part[kAnalysisBlocks + 3 w + {0, 1, 2}] (axial energy,
+// strain sum, max |strain|); strain[e] per bond.  bonds: (i, j, wx, wy) in
+// the network's device node indices, edge order.
+constexpr int kAnalysisBlocks = 256;
+int launch_analysis(const double2 *x, const double2 *x0, int n, double miny, double shear,
+                    double axial, const int4 *bonds, int ne, double W, double H, double shift,
+                    double *strain, double *part, void *stream);
+
 }  // namespace nr

@@ -159,20 +159,6 @@ __device__ inline void st_state32(bool sh, float2 *p, float2 v)
     }
 }
 
-__device__ inline void st_shared_f64(double *p, double v)
-{
-    __hip_atomic_store(reinterpret_cast<unsigned long long *>(p),
-                       (unsigned long long)__double_as_longlong(v), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-
-__device__ inline double ld_shared_f64(const double *p)
-{
-    return __longlong_as_double((long long)__hip_atomic_load(
-        reinterpret_cast<unsigned long long *>(const_cast<double *>(p)), __ATOMIC_RELAXED,
-        __HIP_MEMORY_SCOPE_AGENT));
-}
-
 // Waits until every memory operation this wave issued has completed.
 __device__ inline void drain_memory()
 {

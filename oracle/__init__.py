@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by
 bench.py's CPU-baseline leg, as the checker.  The product never imports it.
-The C sources (oracle.c, linear.c, oracle.h) cite the reference lines they
-restate.
+The C sources (oracle.c, linear.c, analysis.c, oracle.h) cite the reference
+lines they restate.
 """
 import ctypes as C
 import os
@@ -104,6 +104,9 @@ def lib():
         L.or_linear_energies.argtypes = [P(OrNet), C.c_double, C.c_double, C.c_double, dp,
                                          C.c_double, C.c_double, C.c_int, dp, dp]
         L.or_linear_energies.restype = None
+        L.or_analyze.argtypes = [P(OrNet), dp, dp, C.c_double, C.c_double, C.c_double,
+                                 C.c_double, dp, dp]
+        L.or_analyze.restype = None
         _lib = L
     return _lib
 
@@ -297,6 +300,18 @@ class Net:
         lib().or_linear_energies(C.byref(self.s), ks, kb, mu, _dp(u), s, c, int(affine),
                                  C.byref(es), C.byref(eb))
         return es.value, eb.value
+
+    def analyze(self, x0, x, shift, shear, axial=0.0, k=1.0):
+        """Post-relax reductions (na_calc, strain_calc, axial_contribution):
+        non-affinity, axial energy, mean and max |bond strain|, bond strains."""
+        x0 = np.ascontiguousarray(x0, dtype=np.float64)
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        st = np.zeros(max(self.s.e, 1))
+        out = np.zeros(4)
+        lib().or_analyze(C.byref(self.s), _dp(x0), _dp(x), shift, shear, axial, k, _dp(st),
+                         _dp(out))
+        return dict(nonaffinity=out[0], axial_energy=out[1], mean_strain=out[2],
+                    max_strain=out[3], strains=st[:self.s.e])
 
     def cg(self, ks, kb, pos, fcut=1e-10, max_iters=1000000, criterion=CRIT_MAX):
         rep = (C.c_double * 4)()

@@ -16,7 +16,9 @@
  *   get_file_lists (src/file_lists.cpp:11-137)
  * and, in linear.c, the small-strain solver of src/sparse_relax.cpp
  * (stiffness assembly :286-428, reduced system :560-605, solve and
- * energies :49-96, :676-881).
+ * energies :49-96, :676-881); in analysis.c, the reductions of
+ * src/na_calc.cpp:308-326, src/strain_calc.cpp:96-120 and
+ * src/axial_contribution.cpp:173-190.
  * plus the extensions the product also carries (Lees-Edwards box, max-force
  * criterion, per-fiber bending stiffness, nonlinear CG).
  *
@@ -138,6 +140,11 @@ int or_linear_response(const or_net *net, double ks, double kb, double mu, doubl
                        int affine, double tol, int max_iter, double *u, double rep[4]);
 void or_linear_energies(const or_net *net, double ks, double kb, double mu, const double *u,
                         double s, double c, int affine, double *e_ss, double *e_bend);
+
+/* Post-relax reductions (oracle/analysis.c): out = {non-affinity, axial
+ * energy, mean bond strain, max |bond strain|}; strain: NULL or e values. */
+void or_analyze(const or_net *net, const double *x0, const double *x, double shift,
+                double shear, double axial, double k, double *strain, double out[4]);
 
 /* Synthetic triangular lattice (spacing 1, odd rows shifted by 1/2), each
  * candidate bond (node*3 + direction) kept when u < p. */
