@@ -99,7 +99,10 @@ typedef struct {
     int team;          /* most workgroups per network, 0 = tuned */
     int device;        /* HIP device ordinal */
     int tiles_per_member; /* a network of T tiles gets a team of about
-                             T / tiles_per_member workgroups, 0 = tuned */
+                             T / tiles_per_member workgroups.  > 0: team
+                             sizes depend on each network alone, so its
+                             results do not depend on the rest of the batch;
+                             0 = tuned for the batch (1, 2, 4 or 8) */
     double mixed_switch;  /* NR_PREC_MIXED: the FP32 stage hands over to FP64
                              when max |F| < mixed_switch x its initial value,
                              0 = default (3e-2) */

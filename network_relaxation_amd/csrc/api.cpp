@@ -56,11 +56,13 @@ using nr::DevBuf;
 using nr::fail;
 using nr::hip_fail;
 
-// Default tiles per team member: the best measured on MI355X for a batch of
-// 512 networks of 1 to 150 tiles of 256 nodes relaxed again (planned from
-// the step counts of its last relax): 5 -> 79, 7 -> 81, 8 -> 84, 10 -> 80,
-// 12 -> 65 networks/s.  A first relax, planned from estimated step counts,
-// takes 6.5 s at 5 and 6.8 s at 8 (one badly estimated network ends last).
+// Largest tiles per team member the default tuning considers: the best
+// measured on MI355X for a batch of 512 networks of 1 to 150 tiles of 256
+// nodes relaxed again (planned from the step counts of its last relax):
+// 5 -> 79, 7 -> 81, 8 -> 84, 10 -> 80, 12 -> 65 networks/s (the cost model
+// prefers 10, which measures slower).  A first relax, planned from estimated
+// step counts, takes 6.5 s at 5 and 6.8 s at 8 (one badly estimated network
+// ends last).
 constexpr int kDefaultTilesPerMember = 8;
 // Default largest team: every resident workgroup of an MI355X at two 256-thread
 // workgroups per CU (a fixed number, so that team sizes do not depend on the
@@ -301,6 +303,27 @@ static int set_team_layout(nr_batch *b, int tiles_per_member)
     return NR_OK;
 }
 
+// Tiles per member when the options leave it open: the one of {1, 2, 4, 8}
+// with the least predicted time for these networks (estimated step counts).
+// A batch of many networks keeps every workgroup busy with small teams (8 is
+// the best measured for 512 networks of 1 to 150 tiles); a few large networks
+// get wide teams instead of leaving workgroups idle (a single network of 1024
+// tiles: teams of 512 rather than 128).  The choice depends on the batch, so
+// a network's reduction order may differ between batches; an explicit
+// tiles_per_member fixes it.
+static int tuned_tiles_per_member(const nr_batch *b)
+{
+    const nr::Plan &P = b->plan;
+    std::vector<int> tiles(P.nets.size());
+    std::vector<double> steps(P.nets.size());
+    for (size_t k = 0; k < P.nets.size(); k++) {
+        tiles[k] = P.nets[k].n_tiles;
+        steps[k] = nr::expected_steps(P.nets[k].n_nodes, P.nets[k].n_bonds);
+    }
+    return nr::tune_tiles_per_member(tiles, steps, b->team_cap, b->resident,
+                                     {1, 2, 4, kDefaultTilesPerMember}, nullptr);
+}
+
 int nr_batch_create(int count, nr_network *const *nets, const double *ks, const double *kb,
                     const nr_batch_opts *opts, nr_batch **out)
 {
@@ -335,7 +358,7 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
     b->resident = per_cu * n_cu;
     b->team_cap = std::min(opts && opts->team > 0 ? opts->team : kDefaultTeamCap, b->resident);
     rc = set_team_layout(b.get(), opts && opts->tiles_per_member > 0 ? opts->tiles_per_member
-                                                               : kDefaultTilesPerMember);
+                                                               : tuned_tiles_per_member(b.get()));
     if (rc) return rc;
 
     // uploads

@@ -97,7 +97,10 @@ __device__ inline double2 ld_state(bool sh, const double2 *p)
 // policy (sc1, what an agent-scope relaxed atomic store of 8 bytes uses):
 // two 8-byte stores to the halves of 16-byte slots would write every 32-byte
 // sector twice.  The store has no outputs; the drain before a record is
-// published waits for it (vmcnt) like any other store.
+// published waits for it (vmcnt) like any other store.  A VALU write to the
+// data registers of a store of more than 64 bits must wait one cycle after
+// it; the compiler cannot see the store inside the asm (and has reused those
+// registers in the very next instruction), so the wait is part of the asm.
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 __device__ inline void st_state(bool sh, double2 *p, double2 v)
 {
@@ -109,7 +112,8 @@ __device__ inline void st_state(bool sh, double2 *p, double2 v)
         w.y = (unsigned)(a >> 32);
         w.z = (unsigned)b;
         w.w = (unsigned)(b >> 32);
-        asm volatile("global_store_dwordx4 %0, %1, off sc1" : : "v"(p), "v"(w) : "memory");
+        asm volatile("global_store_dwordx4 %0, %1, off sc1\n\ts_nop 1" : : "v"(p), "v"(w)
+                     : "memory");
     } else {
         *p = v;
     }

@@ -73,6 +73,25 @@ def test_cg_ramp_equals_sequential_and_step_limit():
     assert r["steps"] == 3 and r["status"] == N.NOT_CONVERGED
 
 
+def test_cg_team_relax_repeats_bit_for_bit():
+    """CG on a team of four workgroups (one tile each) relaxed three times:
+    identical steps, energies and positions.  Team members exchange state
+    through 16-byte stores; a store whose data registers are overwritten
+    too early would show here as a run-to-run difference."""
+    _say("case cg team repeat")
+    n = N.Network.lattice(32, 0.68, 7, True).triples().fiber_kappa(1e-4, 1e-2, 7)
+    out = []
+    for _ in range(3):
+        b = N.Batch([n], 1.0, 1.0, tiles_per_member=1)
+        b.shear(0, 0.01)
+        r = b.relax(N.relax_params(fcut=1e-10, method=N.METHOD_CG))[0]
+        assert r["status"] == N.CONVERGED
+        out.append(((r["steps"], r["es"], r["eb"]), b.positions(0)))
+    for key, pos in out[1:]:
+        assert key == out[0][0]
+        np.testing.assert_array_equal(pos, out[0][1])
+
+
 def test_one_method_per_relax_call():
     _say("case method mix rejected")
     nets = [N.Network.lattice(12, 0.7, s, True).triples() for s in (1, 2)]

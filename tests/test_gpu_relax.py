@@ -70,12 +70,12 @@ def test_batch_alone_equals_in_mix():
     _say("case batch alone vs mix")
     nets = [N.Network.lattice(L, 0.68, 100 + L, True).triples() for L in (12, 20, 16)]
     prm = N.relax_params(dt_max=0.2, fcut=1e-10)
-    mix = N.Batch(nets, 1.0, 3e-3)
+    mix = N.Batch(nets, 1.0, 3e-3, tiles_per_member=2)
     mix.shear(-1, 0.02)
     rm = mix.relax(prm)
     for k, net in enumerate(nets):
         alone = N.Batch([N.Network.lattice((12, 20, 16)[k], 0.68, 100 + (12, 20, 16)[k], True).triples()],
-                        1.0, 3e-3)
+                        1.0, 3e-3, tiles_per_member=2)
         alone.shear(0, 0.02)
         ra = alone.relax(prm)[0]
         assert ra["steps"] == rm[k]["steps"]
@@ -87,7 +87,7 @@ def test_team_kernel_matches_single_workgroup():
     _say("case team vs single workgroup")
     o, n = _pair(48, 0.7, 9, True, 1e-2)
     prm = N.relax_params(dt_max=0.2, fcut=1e-10)
-    team = N.Batch([n], 1.0, 1e-2, tile_nodes=256)      # 9 tiles -> team of 9
+    team = N.Batch([n], 1.0, 1e-2, tile_nodes=256)      # 9 tiles -> team of 8
     one = N.Batch([n], 1.0, 1e-2, tile_nodes=256, team=1)
     for b in (team, one):
         b.shear(0, 0.01)
@@ -132,7 +132,10 @@ def test_config1_plain_fp64():
     assert r["status"] == N.CONVERGED and r["fmax"] < 1e-10
     et, eo = r["es"] + r["eb"], ro["es"] + ro["eb"]
     assert abs(et - eo) <= 1e-12 * eo
-    assert abs(r["es"] - ro["es"]) <= 1e-12 * eo and abs(r["eb"] - ro["eb"]) <= 1e-12 * eo
+    # the split moves at first order in the residual position error (stretch
+    # and bend forces balance, neither vanishes), the total only at second
+    # order: two minimisers stopped at |F|max < 1e-10 agree on it to ~1e-10
+    assert abs(r["es"] - ro["es"]) <= 1e-9 * eo and abs(r["eb"] - ro["eb"]) <= 1e-9 * eo
 
 
 def test_fiber_kappa_relax_matches_oracle():
@@ -226,11 +229,11 @@ def test_mixed_batch_alone_equals_in_mix():
     sizes = (12, 28, 18)
     mk = lambda L: N.Network.lattice(L, 0.66, 300 + L, True).triples().fiber_kappa(1e-4, 1e-2, L)
     prm = N.relax_params(dt_max=0.2, fcut=1e-9)
-    mix = N.Batch([mk(L) for L in sizes], 1.0, 1.0, precision=N.PREC_MIXED)
+    mix = N.Batch([mk(L) for L in sizes], 1.0, 1.0, precision=N.PREC_MIXED, tiles_per_member=2)
     mix.shear(-1, 0.05)
     rm = mix.relax(prm)
     for k, L in enumerate(sizes):
-        alone = N.Batch([mk(L)], 1.0, 1.0, precision=N.PREC_MIXED)
+        alone = N.Batch([mk(L)], 1.0, 1.0, precision=N.PREC_MIXED, tiles_per_member=2)
         alone.shear(0, 0.05)
         ra = alone.relax(prm)[0]
         assert (ra["steps"], ra["es"], ra["eb"]) == (rm[k]["steps"], rm[k]["es"], rm[k]["eb"])

@@ -147,11 +147,11 @@ def test_multi_member_teams_alone_equal_in_mix():
     sizes = (48, 64, 80, 96, 112, 128)
     mk = lambda k: N.Network.lattice(sizes[k], 0.66, 70 + k, True).triples()
     prm = N.relax_params(dt_max=0.3, fcut=1e-9)
-    mix = N.Batch([mk(k) for k in range(len(sizes))], 1.0, 2e-3)
+    mix = N.Batch([mk(k) for k in range(len(sizes))], 1.0, 2e-3, tiles_per_member=4)
     mix.shear(-1, 0.04)
     rm = mix.relax(prm)
     for k in range(len(sizes)):
-        alone = N.Batch([mk(k)], 1.0, 2e-3)
+        alone = N.Batch([mk(k)], 1.0, 2e-3, tiles_per_member=4)
         alone.shear(0, 0.04)
         ra = alone.relax(prm)[0]
         assert _key(ra) == _key(rm[k]), k
