@@ -38,6 +38,12 @@ def progress(msg):
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per relax launch of the default workload (512 networks, seed 2024,
+# L 16..192, |F|max < 1e-9, tuned shapes; a relax planned from the previous
+# one's step counts), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
+# scaled by what those counters read for a 1 GiB device copy (fetch x 2.0,
+# write x 1.0): 12,135,584,362 KiB x 2 + 6,454,443,609 KiB.
+PROFILED_TRAFFIC_BYTES = (12135584362 * 2 + 6454443609) * 1024.0
 FIRE = dict(alpha0=0.1, f_alpha=0.99, dt_max=0.5, f_inc=1.1, f_dec=0.5, nmin=5, mass=1.0)
 
 
@@ -174,6 +180,9 @@ def main():
         total_bytes += s32 * alg_bytes32(n_, e_, t_) + s64 * alg_bytes(n_, e_, t_)
         node_steps += reps[k]["steps"] * n_
     achieved = total_bytes / (kernel_ms * 1e-3) / 1e9
+    profiled = (args.networks, args.lmin, args.lmax, args.seed, args.fcut, args.tile, args.team,
+                args.tpm, args.switch) == (512, 16, 192, 2024, 1e-9, 0, 0, 0, 0.0)
+    traffic = PROFILED_TRAFFIC_BYTES if profiled else None
     e_mixed = np.array([r["es"] + r["eb"] for r in reps])
     first_ms = cold_ms[0] if cold_ms else None
     del mixed
@@ -259,7 +268,7 @@ def main():
         "speedup_vs_plain": (value / plain["networks_per_s"]) if plain else None,
         "served": served,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
                      "kernel": "relax_kernel<256,1>", "kernel_ms": kernel_ms,
                      "alg_bytes_per_launch": total_bytes},
         "cpu_baseline": cpu,
