@@ -17,7 +17,7 @@ network per launch, FP64) on a sample, a served stream (Poisson arrivals of
 strain-ramp jobs through the serving loop, p50/p99 latency), the relax
 kernel's algorithmic HBM rate against MI355X's 8 TB/s, and the CPU FP64 oracle
 on the host cores over a bounded sample (FIRE, and beside it the reference's
-small-strain linear solve).  Prints one JSON line.
+small-strain linear solve, also timed on the GPU).  Prints one JSON line.
 """
 import argparse
 import json
@@ -104,6 +104,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--no-linear", action="store_true", help="skip the GPU linear solves")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB after the timed work, as "
@@ -228,6 +229,8 @@ def main():
     if args.served > 0:
         served = served_leg(N, args)
 
+    lin = None if args.no_linear else gpu_linear(N, specs, linear_sample(specs, args))
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(specs, reps, args)
@@ -272,6 +275,7 @@ def main():
                      "kernel": "relax_kernel<256,1>", "kernel_ms": kernel_ms,
                      "alg_bytes_per_launch": total_bytes},
         "cpu_baseline": cpu,
+        "linear_response_gpu": lin,
     }
     if args.calibrate_copy:  # after the timed work, for the counter passes
         out["copy_calibration"] = calibrate_copy(args.calibrate_copy)
@@ -541,7 +545,35 @@ def cpu_baseline(specs, reps, args):
             "sample": f"{len(idx)} sampled networks x {per_net_steps} FIRE steps on {cores} "
                       f"threads ({wall:.1f} s): {rate:.3e} node-steps/s, divided by the batch's "
                       f"measured {node_steps_per_net:.3e} node-steps per relaxed network",
-            "linear": cpu_linear(specs, idx[:4], cores)}
+            "linear": cpu_linear(specs, linear_sample(specs, args), cores)}
+
+
+def linear_sample(specs, args):
+    """Four networks of the batch for the small-strain linear solves (the
+    same ones on the CPU and on the GPU)."""
+    rng = np.random.RandomState(args.seed + 3)
+    return sorted(rng.choice(len(specs), size=min(4, len(specs)), replace=False))
+
+
+def gpu_linear(N, specs, idx):
+    """The small-strain linear solve (nr_linear_response: CG on the device,
+    stiffness assembled on the host) of the linear CPU baseline's networks at
+    gamma = 0.01."""
+    out = []
+    t0 = time.perf_counter()
+    for k in idx:
+        s = specs[k]
+        n = N.Network.lattice(s["L"], s["p"], s["seed"], True).triples()
+        n.fiber_kappa(1e-5, 1e-2, s["kseed"])
+        t = time.perf_counter()
+        r = N.linear_response(n, 1.0, 1.0, 0.01, max_iter=20000)
+        out.append(dict(L=s["L"], seconds=time.perf_counter() - t, device_ms=r["device_ms"],
+                        cg_iters=r["iters"], residual=r["residual"]))
+    wall = time.perf_counter() - t0
+    progress(f"linear solves on the GPU done ({wall:.1f} s)")
+    return {"solves_per_s": len(out) / sum(o["seconds"] for o in out), "gamma": 0.01,
+            "cg_iter_cap": 20000, "networks": out,
+            "note": "host assembly + device CG per call; seconds include the assembly"}
 
 
 def cpu_linear(specs, idx, cores):

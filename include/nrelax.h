@@ -209,6 +209,35 @@ int nr_batch_analyze(nr_batch *b, int index, double shear_strain, double axial_s
                      double k_axial, nr_analysis *out, double *bond_strains);
 /* Device time of the last relax launch in milliseconds. */
 double nr_batch_last_ms(const nr_batch *b);
+
+/* ---- Small-strain linear response (sparse_relax.cpp) ------------------- */
+/* Replaces do_relaxation_run of sparse_relax.cpp (:676-881) for one load step:
+ * the stiffness assembled from the reference's element blocks (:286-428),
+ * the reduced system over the free DOFs K_rr w = -K_rN bc (get_lhs_rhs
+ * :560-605) solved on the current HIP device by conjugate gradients from
+ * w = 0 (the reference factorises it by sparse QR; CG gives the
+ * minimum-norm solution: floppy modes and bond-less nodes do not move), to
+ * |K_rr w + K_rN bc| <= tol x the imposed field's force scale or max_iter
+ * iterations.  bc: with affine != 0 the affine field of shear and
+ * compression strains (affine_displacement :527-537), else the top row
+ * displaced by (shear, -compression) (add_displacement :513-523; reference
+ * mode only).  In a Lees-Edwards box every node is free and images across
+ * the y boundary carry the field's offset.  u (NULL or 2N doubles) receives
+ * bc + w; the report the linearised energies (get_stretch_shear_energy
+ * :49-70 with gel modulus mu, get_bending_energy :73-96) and the residual
+ * 2-norm (the reference's "Residual RMS force" line, :841-850).  Bending
+ * (kb > 0 or per-fiber kappa) needs nr_network_build_triples first. */
+typedef struct {
+    double e_stretch;   /* stretch + gel shear energy */
+    double e_bend;
+    double residual;
+    int iterations;
+    int status;         /* NR_CONVERGED or NR_NOT_CONVERGED */
+    double device_ms;   /* device time of the iterations */
+} nr_linear_report;
+int nr_linear_response(const nr_network *net, double ks, double kb, double mu, double shear,
+                       double compression, int affine, double tol, int max_iter, double *u,
+                       nr_linear_report *rep);
 /* Bytes one FIRE step moves per the algorithmic count for this batch. */
 int nr_batch_counts(const nr_batch *b, int index, int64_t *nodes,
                     int64_t *bonds, int64_t *triples);

@@ -79,6 +79,11 @@ class Analysis(C.Structure):
                 ("mean_bond_strain", C.c_double), ("max_bond_strain", C.c_double)]
 
 
+class LinearReport(C.Structure):
+    _fields_ = [("e_stretch", C.c_double), ("e_bend", C.c_double), ("residual", C.c_double),
+                ("iterations", C.c_int), ("status", C.c_int), ("device_ms", C.c_double)]
+
+
 class NRError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"nrelax error {code}: {msg}")
@@ -119,6 +124,9 @@ def lib():
         L.nr_batch_eval.argtypes = [vp, C.c_int, dp, dp]
         L.nr_batch_analyze.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double,
                                        P(Analysis), dp]
+        L.nr_linear_response.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
+                                         C.c_double, C.c_int, C.c_double, C.c_int, dp,
+                                         P(LinearReport)]
         L.nr_batch_last_ms.argtypes = [vp]
         L.nr_batch_last_ms.restype = C.c_double
         L.nr_batch_counts.argtypes = [vp, C.c_int, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
@@ -429,3 +437,19 @@ def relaxation_batch(batch_file, strain_mode=True, precision=PREC_FP64, tile_nod
     opts = BatchOpts(precision, 0, tile_nodes, 0, 0, 0, 0.0)
     return lib().nr_relaxation_batch(os.fsencode(batch_file), 1 if strain_mode else 0,
                                      C.byref(opts))
+
+
+def linear_response(net, ks, kb, shear, compression=0.0, mu=0.0, affine=True, tol=1e-12,
+                    max_iter=1000000, displacements=False):
+    """Small-strain linear response of one network on the current device
+    (nr_linear_response): linearised energies, residual, CG iterations, and
+    with displacements=True the displacement field u (2N)."""
+    rep = LinearReport()
+    u = np.zeros(2 * net.info().n_nodes) if displacements else None
+    _check(lib().nr_linear_response(net.h, ks, kb, mu, shear, compression, int(affine), tol,
+                                    max_iter, _dp(u) if displacements else None, C.byref(rep)))
+    out = dict(es=rep.e_stretch, eb=rep.e_bend, residual=rep.residual,
+               iters=rep.iterations, status=rep.status, device_ms=rep.device_ms)
+    if displacements:
+        out["u"] = u
+    return out

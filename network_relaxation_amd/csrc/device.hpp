@@ -147,6 +147,18 @@ int launch_zero(void *p, size_t bytes, void *stream);
 // strain sum, max |strain|); strain[e] per bond.  bonds: (i, j, wx, wy) in
 // the network's device node indices, edge order.
 constexpr int kAnalysisBlocks = 256;
+
+// Small-strain linear solve (linear_kernels.hip): one CG iteration on
+// K_rr w = b (CSR over n DOFs, rows of clamped DOFs masked), scalars in sc
+// (see there), kLinBlocks partials of p.q then r.r in part[0 .. 2 kLinBlocks).
+constexpr int kLinBlocks = 512;
+int launch_cg_iteration(int n, const int *rowptr, const int *col, const double *val,
+                        const unsigned char *free_dof, double *w, double *r, double *p, double *q,
+                        double *part, double *sc, int max_iter, void *stream);
+// y = P K x (clamped rows 0), partials of x.y in part[0 .. kLinBlocks).
+int launch_spmv(int n, const int *rowptr, const int *col, const double *val,
+                const unsigned char *free_dof, const double *x, double *y, double *part,
+                void *stream);
 int launch_analysis(const double2 *x, const double2 *x0, int n, double miny, double shear,
                     double axial, const int4 *bonds, int ne, double W, double H, double shift,
                     double *strain, double *part, void *stream);

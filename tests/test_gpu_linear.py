@@ -1,0 +1,58 @@
+"""Device small-strain linear solve (nr_linear_response) against the oracle's
+restatement of the reference's sparse_relax.cpp (itself pinned by
+tests/test_linear.py): energies, residual and the minimum-norm displacement
+field; and the full-lattice closed form on the device."""
+import math
+
+import numpy as np
+import pytest
+
+import network_relaxation_amd as N
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("L,p,seed,le,kappa,mu", [
+    (24, 0.8, 3, True, None, 0.0),
+    (20, 0.75, 4, False, (1e-4, 1e-2, 5), 0.0),
+    (16, 0.7, 9, True, (1e-3, 1e-1, 2), 0.05),
+])
+def test_device_linear_matches_oracle(L, p, seed, le, kappa, mu):
+    print(f"linear L={L} p={p} le={le} mu={mu}", flush=True)
+    n = N.Network.lattice(L, p, seed, le).triples()
+    o = O.Net.lattice(L, p, seed, le).triples()
+    kb = 1e-2
+    if kappa:
+        n.fiber_kappa(*kappa)
+        o.fiber_kappa(*kappa)
+        kb = 0.0
+    g = 0.01
+    dev = N.linear_response(n, 1.0, kb, g, mu=mu, displacements=True)
+    ref = o.linear_response(1.0, kb, g, mu=mu)
+    assert dev["status"] == N.CONVERGED and ref["rc"] == 0
+    e_d, e_r = dev["es"] + dev["eb"], ref["es"] + ref["eb"]
+    assert e_d == pytest.approx(e_r, rel=1e-9)
+    assert dev["es"] == pytest.approx(ref["es"], rel=1e-8)
+    assert dev["eb"] == pytest.approx(ref["eb"], rel=1e-8, abs=1e-14)
+    assert dev["residual"] < 1e-10
+    scale = np.abs(ref["u"]).max()
+    assert np.abs(dev["u"] - ref["u"]).max() < 1e-6 * scale
+    assert abs(dev["iters"] - ref["iters"]) <= max(3, ref["iters"] // 50)
+
+
+def test_device_linear_full_lattice_closed_form():
+    L, g = 16, 0.02
+    n = N.Network.lattice(L, 1.0, 1, True).triples()
+    r = N.linear_response(n, 1.0, 1e-2, g, displacements=True)
+    assert r["status"] == N.CONVERGED and r["iters"] == 0
+    assert r["es"] == pytest.approx(0.5 * (g * math.sqrt(3.0) / 4.0) ** 2 * 2 * L * L, rel=1e-12)
+    assert r["eb"] < 1e-25
+
+
+def test_device_linear_repeats_bit_for_bit():
+    n = N.Network.lattice(24, 0.7, 6, False).triples().fiber_kappa(1e-4, 1e-2, 3)
+    a = N.linear_response(n, 1.0, 0.0, 0.01, displacements=True)
+    b = N.linear_response(n, 1.0, 0.0, 0.01, displacements=True)
+    assert (a["iters"], a["es"], a["eb"]) == (b["iters"], b["es"], b["eb"])
+    np.testing.assert_array_equal(a["u"], b["u"])

@@ -78,3 +78,14 @@ def test_reader_errors_match_reference_codes(tmp_path):
     with pytest.raises(N.NRError) as e:
         N.Network.read(str(tmp_path / "absent.dat"))
     assert e.value.code == N.E_OPEN
+
+
+def test_linear_response_argument_errors():
+    """Rejected before any device work: a periodic box has no top row, and
+    bending needs the triples."""
+    le = N.Network.lattice(8, 0.8, 1, True).triples()
+    with pytest.raises(N.NRError):
+        N.linear_response(le, 1.0, 1e-3, 0.01, affine=False)
+    bare = N.Network.lattice(8, 0.8, 1, False)
+    with pytest.raises(N.NRError):
+        N.linear_response(bare, 1.0, 1e-3, 0.01)
