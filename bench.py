@@ -315,7 +315,7 @@ def single_leg(N, args):
 def ramp_leg(N, args):
     """L=512 (p 0.6, kappa 1e-4) Lees-Edwards network, shear ramp in
     increments of 0.005 (100 reach 0.5; 20 by default), each relaxed to
-    |F|max < fcut from the previous state, FIRE and CG, ten increments per
+    |F|max < fcut from the previous state, FIRE and CG, five increments per
     device launch (nr_batch_ramp)."""
     ninc = args.increments or 20
     n = N.Network.lattice(512, 0.6, 2, True).triples()
@@ -330,8 +330,8 @@ def ramp_leg(N, args):
                              max_steps=1000000, **FIRE)
         t = time.perf_counter()
         reps, kms = [], 0.0
-        for c0 in range(0, ninc, 10):  # ten increments per launch, a progress line each
-            reps += b.ramp(incs[c0:c0 + 10], prm)[0]
+        for c0 in range(0, ninc, 5):  # five increments per launch, a progress line each
+            reps += b.ramp(incs[c0:c0 + 5], prm)[0]
             kms += b.last_ms()
             progress(f"ramp {name}: {len(reps)}/{ninc} increments, "
                      f"{sum(r['steps'] for r in reps)} steps")
