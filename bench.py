@@ -541,11 +541,26 @@ def cpu_baseline(specs, reps, args):
     node_steps_per_net = np.mean([reps[k]["steps"] * specs[k]["L"] ** 2 for k in range(len(specs))])
     progress(f"cpu baseline: FIRE sample done ({wall:.1f} s)")
     return {"value": rate / node_steps_per_net, "unit": "networks/s", "cores": cores,
-            "kind": "port",
+            "kind": "port", "host": host_cpu(),
+            "node_steps_per_s_per_thread": rate / cores,
             "sample": f"{len(idx)} sampled networks x {per_net_steps} FIRE steps on {cores} "
                       f"threads ({wall:.1f} s): {rate:.3e} node-steps/s, divided by the batch's "
                       f"measured {node_steps_per_net:.3e} node-steps per relaxed network",
             "linear": cpu_linear(specs, linear_sample(specs, args), cores)}
+
+
+def host_cpu():
+    """The host's logical CPU count and CPU model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"nproc": os.cpu_count(), "model": model}
 
 
 def linear_sample(specs, args):
