@@ -148,13 +148,16 @@ int launch_zero(void *p, size_t bytes, void *stream);
 // the network's device node indices, edge order.
 constexpr int kAnalysisBlocks = 256;
 
-// Small-strain linear solve (linear_kernels.hip): one CG iteration on
-// K_rr w = b (CSR over n DOFs, rows of clamped DOFs masked), scalars in sc
-// (see there), kLinBlocks partials of p.q then r.r in part[0 .. 2 kLinBlocks).
+// Small-strain linear solve (linear_kernels.hip): iteration k of a chunk
+// (k from 0, chunks of an even length) of CG on K_rr w = b (CSR over n DOFs,
+// rows of clamped DOFs masked), scalars in sc (see there), kLinBlocks
+// partials of p.q then r.r in part[0 .. 2 kLinBlocks); a chunk ends with
+// launch_cg_chunk_end.
 constexpr int kLinBlocks = 512;
 int launch_cg_iteration(int n, const int *rowptr, const int *col, const double *val,
                         const unsigned char *free_dof, double *w, double *r, double *p, double *q,
-                        double *part, double *sc, int max_iter, void *stream);
+                        double *part, double *sc, int k, int max_iter, void *stream);
+int launch_cg_chunk_end(double *sc, int chunk, void *stream);
 // y = P K x (clamped rows 0), partials of x.y in part[0 .. kLinBlocks).
 int launch_spmv(int n, const int *rowptr, const int *col, const double *val,
                 const unsigned char *free_dof, const double *x, double *y, double *part,

@@ -264,14 +264,14 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
     HIP_TRY(d_p.upload(b0));
     HIP_TRY(d_q.upload(zero));
     HIP_TRY(d_part.upload(std::vector<double>(2 * nr::kLinBlocks, 0.0)));
-    HIP_TRY(d_sc.upload({rr0, 0.0, 0.0, done0 ? 1.0 : 0.0, 0.0, thr2}));
+    HIP_TRY(d_sc.upload({rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0}));
 
     hipStream_t st = nullptr;
     hipEvent_t e0 = nullptr, e1 = nullptr;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     int rc = NR_OK;
-    double sc[6] = {rr0, 0, 0, done0 ? 1.0 : 0.0, 0, thr2};
+    double sc[8] = {rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0};
     float ms = 0.0f;
     auto cleanup = [&]() {
         if (exec) (void)hipGraphExecDestroy(exec);
@@ -285,13 +285,14 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
     if (e == hipSuccess) e = hipEventCreate(&e1);
     // a chunk of iterations as one graph (kernels only); replays stop doing
     // work once the device raises its done flag
-    constexpr int kChunk = 32;
+    constexpr int kChunk = 32;  // even: the parity of iteration k repeats per chunk
     if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
     if (e == hipSuccess) {
         for (int c = 0; c < kChunk && e == hipSuccess; c++)
             e = (hipError_t)nr::launch_cg_iteration(n2, d_rowptr.p, d_col.p, d_val.p, d_free.p,
                                                     d_w.p, d_r.p, d_p.p, d_q.p, d_part.p, d_sc.p,
-                                                    max_iter, st);
+                                                    c, max_iter, st);
+        if (e == hipSuccess) e = (hipError_t)nr::launch_cg_chunk_end(d_sc.p, kChunk, st);
         hipError_t e2 = hipStreamEndCapture(st, &graph);
         if (e == hipSuccess) e = e2;
     }
@@ -346,8 +347,8 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
     rep->e_stretch = es;
     rep->e_bend = eb;
     rep->residual = std::sqrt(res);
-    rep->iterations = (int)sc[4];
-    rep->status = !(sc[0] > thr2) ? NR_CONVERGED : NR_NOT_CONVERGED;
+    rep->iterations = (int)sc[6];
+    rep->status = !(sc[7] > thr2) ? NR_CONVERGED : NR_NOT_CONVERGED;
     rep->device_ms = ms;
     if (u) std::copy(uf.begin(), uf.end(), u);
     return rc;
