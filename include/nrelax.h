@@ -198,7 +198,9 @@ int nr_batch_eval(nr_batch *b, int index, double *force, double *energies);
  *                    (l - l0) / l0: get_strains, strain_calc.cpp:96-120
  * bond_strains (NULL or one double per bond) receives the strains in the
  * network's edge order.  In a Lees-Edwards box the image of a bond's second
- * node across the y boundary moves with the box shift. */
+ * node across the y boundary moves with the box shift.  The bond list is read
+ * from the network given to nr_batch_create, which must still exist (the
+ * first call per network uploads it; later calls reuse the upload). */
 typedef struct {
     double nonaffinity;
     double axial_energy;

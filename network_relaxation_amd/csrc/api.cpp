@@ -92,7 +92,8 @@ struct nr_batch {
     std::vector<std::unique_ptr<LaunchDev>> launches;
     std::vector<int> team_of_net;
     nr::Plan plan;
-    std::vector<const nr::Network *> nets;  // borrowed for heights / miny
+    // borrowed: read while the batch is created, and by nr_batch_analyze
+    std::vector<const nr::Network *> nets;
     std::vector<nr::NetDev> hnet;
     std::vector<double> ks, kb;
     DevBuf<double2> x[2], v[2], f[2], x_init;
