@@ -301,6 +301,11 @@ typedef struct {
     int n_nodes;
     double t_submit, t_admit, t_done;     /* nr_server_clock() seconds */
     double e_stretch, e_bend, max_force;  /* after the last increment */
+    /* the post-relax reductions of nr_batch_analyze at the job's final state,
+     * computed on the device when the job retires: non-affinity against the
+     * affine field of the applied increments (sum of xdisp and of ydisp over
+     * the y range), axial energy with k = ks, mean and max |bond strain| */
+    double nonaffinity, axial_energy, mean_bond_strain, max_bond_strain;
 } nr_job_result;
 
 typedef struct {

@@ -65,7 +65,9 @@ class JobResult(C.Structure):
     _fields_ = [("job", C.c_int), ("status", C.c_int), ("error", C.c_int), ("n_inc", C.c_int),
                 ("steps", C.c_int), ("n_nodes", C.c_int), ("t_submit", C.c_double),
                 ("t_admit", C.c_double), ("t_done", C.c_double), ("e_stretch", C.c_double),
-                ("e_bend", C.c_double), ("max_force", C.c_double)]
+                ("e_bend", C.c_double), ("max_force", C.c_double),
+                ("nonaffinity", C.c_double), ("axial_energy", C.c_double),
+                ("mean_bond_strain", C.c_double), ("max_bond_strain", C.c_double)]
 
 
 class ServerInfo(C.Structure):

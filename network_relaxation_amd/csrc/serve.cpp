@@ -113,6 +113,7 @@ struct Job {
     int status = NR_CONVERGED, error = NR_OK, steps = 0, n_done = 0;
     std::vector<nr_report> reports;
     std::vector<double> pos;
+    double an[4] = {0.0, 0.0, 0.0, 0.0};  // post-relax reductions (nr_job_result)
 };
 
 // Device tables of one launch of a slice (reused across slices).
@@ -496,6 +497,55 @@ int launch_slice(nr_server *s)
     return NR_OK;
 }
 
+// Post-relax reductions of a finished job (as nr_batch_analyze) from its
+// resident state: the input geometry and the bond list go up, the reductions
+// come back.
+int analyze_job(nr_server *s, Job &j, const nr::NetDev &nd)
+{
+    const nr::Network &N = j.net;
+    const nr::PlanNet &pn = j.plan.nets[0];
+    const int ne = N.e();
+    std::vector<int4> bonds(std::max(ne, 1));
+    for (int e = 0; e < ne; e++)
+        bonds[e] = make_int4(pn.iperm[N.bonds[2 * e]], pn.iperm[N.bonds[2 * e + 1]],
+                             N.bwrap[2 * e], N.bwrap[2 * e + 1]);
+    std::vector<double2> x0(std::max(N.n, 1));
+    for (int d = 0; d < N.n; d++)
+        x0[d] = make_double2(N.pos[2 * pn.perm[d]], N.pos[2 * pn.perm[d] + 1]);
+    DevBuf<int4> d_bonds;
+    DevBuf<double2> d_x0;
+    DevBuf<double> strain, part;
+    HIP_TRY(d_bonds.upload(bonds));
+    HIP_TRY(d_x0.upload(x0));
+    HIP_TRY(strain.alloc(std::max(ne, 1)));
+    HIP_TRY(part.alloc(4 * nr::kAnalysisBlocks));
+    const double yr = N.height > 0 ? N.height : N.maxy - N.miny;
+    double gx = 0.0, gy = 0.0;
+    for (int i = 0; i < j.n_done; i++) {
+        gx += j.incs[i].x;
+        gy += j.incs[i].y;
+    }
+    HIP_TRY((hipError_t)nr::launch_analysis(s->x[nd.parity].p + j.node_off, d_x0.p, N.n, N.miny,
+                                            gx / yr, gy / yr, d_bonds.p, ne, N.width, N.height,
+                                            nd.le_shift, strain.p, part.p, s->stream));
+    std::vector<double> hp(4 * nr::kAnalysisBlocks);
+    HIP_TRY(hipMemcpyAsync(hp.data(), part.p, sizeof(double) * hp.size(), hipMemcpyDeviceToHost,
+                           s->stream));
+    HIP_TRY(hipStreamSynchronize(s->stream));
+    double na = 0.0, ax = 0.0, sum = 0.0, mx = 0.0;
+    for (int w = 0; w < nr::kAnalysisBlocks; w++) {
+        na += hp[w];
+        ax += hp[nr::kAnalysisBlocks + 3 * w];
+        sum += hp[nr::kAnalysisBlocks + 3 * w + 1];
+        mx = std::max(mx, hp[nr::kAnalysisBlocks + 3 * w + 2]);
+    }
+    j.an[0] = N.n > 0 ? na / N.n : 0.0;
+    j.an[1] = j.ks * ax;
+    j.an[2] = ne > 0 ? sum / ne : 0.0;
+    j.an[3] = mx;
+    return NR_OK;
+}
+
 // Waits for the launch in flight and retires the jobs it finished.
 int retire(nr_server *s)
 {
@@ -550,6 +600,8 @@ int retire(nr_server *s)
                 j.pos[2 * perm[d] + 1] = xs[d].y;
             }
         }
+        const int arc = analyze_job(s, j, nd);
+        if (arc) return arc;
         j.t_done = now;
         j.state = kDone;
         release_job(s, j);
@@ -747,6 +799,10 @@ int nr_server_result(nr_server *s, int job, nr_job_result *res, nr_report *repor
             res->e_bend = j.reports.back().e_bend;
             res->max_force = j.reports.back().max_force;
         }
+        res->nonaffinity = j.an[0];
+        res->axial_energy = j.an[1];
+        res->mean_bond_strain = j.an[2];
+        res->max_bond_strain = j.an[3];
     }
     if (reports) std::copy(j.reports.begin(), j.reports.end(), reports);
     if (pos && !j.pos.empty()) std::copy(j.pos.begin(), j.pos.end(), pos);

@@ -101,6 +101,17 @@ def test_served_job_equals_one_launch_ramp(precision):
         ramp = b.ramp(incs, prm)[0]
         assert [_key(x) for x in r["reports"]] == [_key(x) for x in ramp]
         np.testing.assert_array_equal(r["pos"], b.positions(0))
+        # the reductions the server attaches equal the batch's of the same state
+        yr = n.info().height
+        gx = gy = 0.0
+        for dx, dy in incs:
+            gx += dx
+            gy += dy
+        a = b.analyze(0, gx / yr, gy / yr, 1.0)
+        assert (r["nonaffinity"], r["axial_energy"], r["mean_bond_strain"],
+                r["max_bond_strain"]) == (a["nonaffinity"], a["axial_energy"],
+                                          a["mean_strain"], a["max_strain"])
+        assert r["nonaffinity"] > 0.0
 
 
 def test_served_alone_equals_in_mix():
