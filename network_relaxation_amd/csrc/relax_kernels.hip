@@ -1,7 +1,9 @@
 // gfx950 kernels of the relax step: one persistent launch relaxes every
-// network of a batch.  Workgroups form teams; a team takes networks from a
-// queue (largest first) and runs FIRE on each to convergence, each member
-// owning a fixed set of spatial tiles.  Per FIRE step and tile:
+// network of a batch.  Workgroups form teams; a team takes networks from its
+// size class's queue (longest expected relax first) and runs FIRE on each to
+// convergence, each member owning a fixed set of spatial tiles; a team whose
+// queue is empty splits into halves that serve the next smaller class.  Per
+// FIRE step and tile:
 //   1. the tile's topology blob is copied into LDS with 16-byte loads (once
 //      per network when the member owns a single tile: it stays resident);
 //   2. own + halo positions are advanced into LDS: the deferred FIRE mix,
@@ -11,9 +13,9 @@
 //      once into LDS (one atan2 per triple);
 //   4. gather: each own node adds its incident force vectors through its CSR
 //      list in element order (no atomics, fixed order), half kick, partials.
-// One team barrier per step combines the partials (P = v.F, |v|^2, |F|^2,
-// max |F_node|^2) in member order, so every member takes the same FIRE
-// decision.  References: run_fire_md large_strain_relax.cu:596-762,
+// One exchange of tagged partial records per step combines the partials
+// (P = v.F, |v|^2, |F|^2, max |F_node|^2) in member order, so every member
+// takes the same FIRE decision.  References: run_fire_md large_strain_relax.cu:596-762,
 // axial_forces :386-414, bending_forces :417-469, energies :472-523,
 // affine_displace :527-537.
 #include <hip/hip_runtime.h>
