@@ -240,6 +240,19 @@ typedef struct {
 int nr_linear_response(const nr_network *net, double ks, double kb, double mu, double shear,
                        double compression, int affine, double tol, int max_iter, double *u,
                        nr_linear_report *rep);
+/* The small-strain batch of the reference program `sp_relax`
+ * (run_relaxation_batch, sparse_relax.cpp:1040-1090, and do_relaxation_run
+ * :676-881): the 4-line batch file (base [tag]; ks kb mu; compression min max
+ * inc; shear min max inc), every <base>*.dat, one device solve per load step
+ * (nr_linear_response to 1e-12 of the force scale), and the reference's
+ * outputs: <m>_<tag>_log.txt ("Residual RMS force: %2.10le" per step, then the
+ * elapsed time), <m>_<tag>_edata.txt ("%12.8lf\t%12.8lf\t%12.10le" compression,
+ * shear, energy; itemize: stretch and bend energies apart) and
+ * <m>_<tag>_<NNN>.disp (the displacement field).  strain_mode (-s) scales the
+ * compression by the y extent and the shear by the x extent, affine = 0 (-z)
+ * displaces the top row only.  The reference's null-space removal (-n) has
+ * nothing to do here: the CG solution carries no null-space component. */
+int nr_linear_batch(const char *batch_file, int strain_mode, int affine, int itemize);
 /* Bytes one FIRE step moves per the algorithmic count for this batch. */
 int nr_batch_counts(const nr_batch *b, int index, int64_t *nodes,
                     int64_t *bonds, int64_t *triples);

@@ -14,6 +14,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnrelax.so")
 CLI_PATH = os.path.join(_HERE, "nrelax")
+LINEAR_CLI_PATH = os.path.join(_HERE, "nrelax_linear")
 
 OK = 0
 E_OPEN, E_HEADER, E_POINTS, E_INDEX, E_EDGES, E_ALLOC, E_ARG, E_HIP, E_TOPO, E_BATCH, E_TIMEOUT = range(1, 12)

@@ -3,6 +3,7 @@
 // (file_lists.cpp:80-137), do_relaxation_run (:792-1010) and
 // relaxation_batch (:1016-1044), with the same output files and formats.
 #include <algorithm>
+#include <cfloat>
 #include <cstdio>
 #include <cstring>
 #include <ctime>
@@ -320,7 +321,156 @@ int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
     return rc;
 }
 
+// parse_batch_file of sparse_relax.cpp (:920-1035).
+int parse_linear_batch(const std::string &path, LinearBatchSpec &b, std::string &err)
+{
+    std::ifstream in(path);
+    std::string line;
+    b = LinearBatchSpec();
+    if (!in.is_open()) {
+        err = "The file " + path + " could not be read.";
+        return NR_E_OPEN;
+    }
+    if (!read_line(in, line)) { err = "Not enough information was found."; return NR_E_BATCH; }
+    std::vector<std::string> tok = split_ws(line);
+    if (tok.empty()) { err = "Not enough information was found."; return NR_E_BATCH; }
+    b.base = tok[0];
+    if (tok.size() > 1) b.tag = tok[1];
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    int nr = std::sscanf(line.c_str(), "%lf %lf %lf", &b.ks, &b.kb, &b.mu);
+    if (nr < 3 || b.ks < 0 || b.kb < 0 || b.mu < 0) { err = "Error: invalid format for mechanical attributes."; return NR_E_BATCH; }
+    double mn, mx, inc;
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
+    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of compression."; return NR_E_BATCH; }
+    b.cmin = mn; b.cmax = mx; b.cinc = inc;
+    if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
+    nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
+    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of shear."; return NR_E_BATCH; }
+    b.smin = mn; b.smax = mx; b.sinc = inc;
+    return NR_OK;
+}
+
+// do_relaxation_run of sparse_relax.cpp (:676-881): for every compression,
+// then shear, of the load grid, one small-strain solve from the input
+// geometry; per load step a residual line in the log, an energy line and a
+// displacement file.  In strain mode the compression is scaled by the y
+// extent and the shear by the x extent (:731-739), then both are divided by
+// the y extent to form the affine field (:785), as the reference does.
+int run_linear_protocol(const JobFiles &jf, const LinearBatchSpec &spec, bool strain_mode,
+                        bool affine, bool itemize, std::string &err)
+{
+    const std::time_t start = std::time(nullptr);
+    nr_network *net = nullptr;
+    int rc = nr_network_read(jf.dat.c_str(), &net);
+    if (rc) {
+        err = nr_last_error();
+        return rc;
+    }
+    struct Guard {
+        nr_network *n = nullptr;
+        ~Guard() { nr_network_free(n); }
+    } g;
+    g.n = net;
+    if (spec.kb > 0) nr_network_build_triples(net);
+    nr_network_info info;
+    nr_network_get_info(net, &info);
+    std::vector<double> pos(2 * (size_t)info.n_nodes);
+    nr_network_get_arrays(net, pos.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                          nullptr);
+    double minx = DBL_MAX, maxx = -DBL_MAX;
+    for (int i = 0; i < info.n_nodes; i++) {
+        minx = std::min(minx, pos[2 * i]);
+        maxx = std::max(maxx, pos[2 * i]);
+    }
+    const double yext = info.maxy - info.miny, xext = maxx - minx;
+    // equal ends: one step (run_relaxation_batch :1081-1082), before scaling
+    double cmin = spec.cmin, cmax = spec.cmax, cinc = spec.cinc;
+    double smin = spec.smin, smax = spec.smax, sinc = spec.sinc;
+    if (smax - smin < kFltTol) sinc = 1e12;
+    if (cmax - cmin < kFltTol) cinc = 1e12;
+    if (strain_mode) {
+        cmin *= yext;
+        cmax *= yext;
+        cinc *= yext;
+        smin *= xext;
+        smax *= xext;
+        sinc *= xext;
+    }
+    if (!(cinc > 0) || !(sinc > 0)) {
+        err = "a load range needs a positive increment";
+        return NR_E_BATCH;
+    }
+    const int num_digits = dec_digits(2 + (int)((cmax - cmin) / cinc) + (int)((smax - smin) / sinc));
+    FILE *lf = std::fopen(jf.log.c_str(), "w");
+    FILE *ef = std::fopen(jf.edata.c_str(), "w");
+    std::vector<double> u(2 * (size_t)info.n_nodes);
+    const int max_iter = 20 * 2 * info.n_nodes + 1000;
+    int disp_count = 1;
+    for (double comp = cmin; rc == NR_OK && comp < cmax + kFltTol; comp += cinc) {
+        for (double shear = smin; shear < smax + kFltTol; shear += sinc) {
+            nr_linear_report rep;
+            const double sf = affine ? shear / yext : shear;
+            const double cf = affine ? comp / yext : comp;
+            rc = nr_linear_response(net, spec.ks, spec.kb, spec.mu, sf, cf, affine ? 1 : 0, 1e-12,
+                                    max_iter, u.data(), &rep);
+            if (rc) {
+                err = nr_last_error();
+                break;
+            }
+            const std::string name = report_name(jf.disp, num_digits, disp_count++, "disp");
+            FILE *df = std::fopen(name.c_str(), "w");
+            if (df) {
+                for (int k = 0; k < info.n_nodes; k++)
+                    std::fprintf(df, "%2.10le\t%2.10le\n", u[2 * k], u[2 * k + 1]);
+                std::fclose(df);
+            }
+            std::fprintf(lf ? lf : stdout, "Residual RMS force: %2.10le\n", rep.residual);
+            FILE *out = ef ? ef : stdout;
+            if (!itemize)
+                std::fprintf(out, "%12.8lf\t%12.8lf\t%12.10le\n", comp, shear,
+                             rep.e_stretch + rep.e_bend);
+            else
+                std::fprintf(out, "%12.8lf\t%12.8lf\t%12.10le\t%12.10le\n", comp, shear,
+                             rep.e_stretch, rep.e_bend);
+        }
+    }
+    if (ef) std::fclose(ef);
+    if (lf) {
+        std::fprintf(lf, "Elapsed time: %d seconds.\n", (int)std::difftime(std::time(nullptr), start));
+        std::fclose(lf);
+    }
+    return rc;
+}
+
 }  // namespace nr
+
+extern "C" int nr_linear_batch(const char *batch_file, int strain_mode, int affine, int itemize)
+{
+    std::string err;
+    nr::LinearBatchSpec b;
+    if (!batch_file) return NR_E_ARG;
+    int rc = nr::parse_linear_batch(batch_file, b, err);
+    if (rc) {
+        std::fprintf(stderr, "%s\nFile parsing failed.\n", err.c_str());
+        return rc;
+    }
+    std::vector<nr::JobFiles> jobs;
+    rc = nr::file_lists(b.base, b.tag, jobs, err);
+    if (rc) {
+        std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
+        return rc;
+    }
+    int first_err = NR_OK;
+    for (const nr::JobFiles &j : jobs) {
+        rc = nr::run_linear_protocol(j, b, strain_mode != 0, affine != 0, itemize != 0, err);
+        if (rc) {
+            std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
+            if (!first_err) first_err = rc;
+        }
+    }
+    return first_err;
+}
 
 extern "C" int nr_relaxation_batch(const char *batch_file, int strain_mode,
                                    const nr_batch_opts *opts)

@@ -36,4 +36,18 @@ std::string report_name(const std::string &base, int digits, int count, const st
 int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
                  const nr_batch_opts *opts, std::string &err);
 
+// Batch file of the reference's small-strain program `sp_relax`
+// (sparse_relax.cpp:920-1035): base [tag]; ks kb mu; compression min max
+// inc; shear min max inc.
+struct LinearBatchSpec {
+    std::string base, tag;
+    double ks = 0, kb = 0, mu = 0;
+    double cmin = 0, cmax = 0, cinc = 0;
+    double smin = 0, smax = 0, sinc = 0;
+};
+int parse_linear_batch(const std::string &path, LinearBatchSpec &b, std::string &err);
+// do_relaxation_run of sparse_relax.cpp (:676-881) on the device solve.
+int run_linear_protocol(const JobFiles &jf, const LinearBatchSpec &b, bool strain_mode,
+                        bool affine, bool itemize, std::string &err);
+
 }  // namespace nr

@@ -61,3 +61,37 @@ def test_cli_protocol_matches_oracle(tmp_path):
         cp = np.loadtxt(cpu_dir / f"netA_t_{k}.pos")
         assert gp.shape == cp.shape
         np.testing.assert_allclose(gp, cp, rtol=0, atol=1e-8)
+
+
+@pytest.mark.parametrize("itemize", [False, True])
+def test_linear_cli_matches_oracle(tmp_path, itemize):
+    """nrelax_linear (the sp_relax drop-in, strain mode) against the oracle's
+    restatement of the same small-strain solve at every load step."""
+    print(f"case linear cli itemize={itemize}", flush=True)
+    net = N.Network.lattice(14, 0.75, 5, False)
+    _write_dat(net, str(tmp_path / "lin.dat"))
+    (tmp_path / "batch.txt").write_text(f"{tmp_path / 'lin'} q\n1.0 0.01 0.0\n0 0 0\n0 0.02 0.01\n")
+    args = [N.LINEAR_CLI_PATH, "-b", str(tmp_path / "batch.txt"), "-s"] + (["-i"] if itemize else [])
+    res = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert res.returncode == 0, res.stderr
+    o = O.Net.read(str(tmp_path / "lin.dat")).triples()
+    yext = o.s.maxy - o.s.miny
+    lines = (tmp_path / "lin_q_edata.txt").read_text().splitlines()
+    log = (tmp_path / "lin_q_log.txt").read_text().splitlines()
+    assert len(lines) == 3 and len(log) == 4
+    assert all(l.startswith("Residual RMS force: ") for l in log[:3])
+    assert log[3].startswith("Elapsed time: ")
+    for k, line in enumerate(lines):
+        cols = line.split("\t")
+        assert len(cols) == (4 if itemize else 3)
+        comp, shear = float(cols[0]), float(cols[1])
+        assert comp == 0.0
+        ref = o.linear_response(1.0, 0.01, shear / yext)
+        if itemize:
+            assert float(cols[2]) == pytest.approx(ref["es"], rel=1e-9, abs=1e-20)
+            assert float(cols[3]) == pytest.approx(ref["eb"], rel=1e-8, abs=1e-20)
+        else:
+            assert float(cols[2]) == pytest.approx(ref["es"] + ref["eb"], rel=1e-9, abs=1e-20)
+        u = np.loadtxt(tmp_path / f"lin_q_{k + 1}.disp")
+        assert u.shape == (o.n, 2)
+        np.testing.assert_allclose(u.reshape(-1), ref["u"], rtol=0, atol=1e-8)

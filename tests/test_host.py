@@ -89,3 +89,19 @@ def test_linear_response_argument_errors():
     bare = N.Network.lattice(8, 0.8, 1, False)
     with pytest.raises(N.NRError):
         N.linear_response(bare, 1.0, 1e-3, 0.01)
+
+
+def test_linear_cli_rejects_bad_batch_files(tmp_path):
+    """nrelax_linear (the sp_relax drop-in) fails on an unreadable or invalid
+    batch file before any device work, with the reference's messages."""
+    import subprocess
+    res = subprocess.run([N.LINEAR_CLI_PATH, "-b", str(tmp_path / "missing.txt")],
+                         capture_output=True, text=True, timeout=60)
+    assert res.returncode == 1 and "could not be read" in res.stderr
+    bad = tmp_path / "bad.txt"
+    bad.write_text("base\n1.0 -1 0\n0 0 0\n0 0.1 0.01\n")
+    res = subprocess.run([N.LINEAR_CLI_PATH, "-b", str(bad)], capture_output=True, text=True,
+                         timeout=60)
+    assert res.returncode == 1 and "mechanical attributes" in res.stderr
+    res = subprocess.run([N.LINEAR_CLI_PATH], capture_output=True, text=True, timeout=60)
+    assert res.returncode == 255
