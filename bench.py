@@ -105,12 +105,15 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-linear", action="store_true", help="skip the GPU linear solves")
+    ap.add_argument("--no-composite", action="store_true",
+                    help="skip the network + gel composite relax")
+    ap.add_argument("--composite-L", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
                     help="also run a device copy of MIB MiB after the timed work, as "
                          "the counter calibration (alone with --steps 0)")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "ramp", "large"],
+                    choices=["batched", "single", "ramp", "large", "composite"],
                     help="batched: the headline (512 uneven networks); single: one "
                          "L=32 network vs the oracle; ramp: L=512, 100 shear increments, "
                          "FIRE and CG; large: L=2048 mixed vs FP64 per increment")
@@ -125,7 +128,8 @@ def main():
             print("bench.py runs on one GPU only (use --gpus 1)", file=sys.stderr)
             sys.exit(2)
         import network_relaxation_amd as N
-        leg = {"single": single_leg, "ramp": ramp_leg, "large": large_leg}[args.workload]
+        leg = {"single": single_leg, "ramp": ramp_leg, "large": large_leg,
+               "composite": composite_leg}[args.workload]
         print(json.dumps(leg(N, args)), flush=True)
         return
 
@@ -230,6 +234,7 @@ def main():
         served = served_leg(N, args)
 
     lin = None if args.no_linear else gpu_linear(N, specs, linear_sample(specs, args))
+    comp = None if args.no_composite else composite_leg(N, args)
 
     cpu = None
     if not args.no_cpu:
@@ -276,6 +281,7 @@ def main():
                      "alg_bytes_per_launch": total_bytes},
         "cpu_baseline": cpu,
         "linear_response_gpu": lin,
+        "composite": comp,
     }
     if args.calibrate_copy:  # after the timed work, for the counter passes
         out["copy_calibration"] = calibrate_copy(args.calibrate_copy)
@@ -481,6 +487,88 @@ def served_leg(N, args):
             "mixed_vs_fp64_energy_rel": {"median": float(np.median(errs)), "max": float(errs.max()),
                                          "n_over_1e-10": int((errs > 1e-10).sum()),
                                          "n": int(len(errs))}}
+
+
+def composite_bytes(n, e, t, f):
+    """Algorithmic bytes of one composite FIRE step (three launches):
+    mesh_elements reads topology + positions and writes slots (bond 72 B,
+    triple 112 B, facet 168 B); mesh_gather reads each slot entry (4 B index
+    + 16 B slot; 2 per bond, 4 per triple, 3 per facet), row, clamp code and
+    twin index (9 B per point) and v, writes F and v (48 B per point);
+    mesh_fire reads v, F, x and writes v, x (80 B per point)."""
+    entries = 2 * e + 4 * t + 3 * f
+    return 72 * e + 112 * t + 168 * f + 20 * entries + n * (9 + 48 + 80)
+
+
+def composite_leg(N, args):
+    """The network + FEM-gel composite (network_mesh_relax.cu): one synthetic
+    composite of side L (ks 1, kb 1e-3, bmod 0.5, smod 0.25, dilation 0.05)
+    sheared by 0.05 from its dilated start and relaxed by FIRE to
+    |F|_rms < 1e-9 (the reference's criterion), FP64.  A first relax (graph
+    capture included) is untimed; the second, from the same start, is timed
+    with HIP events.  CPU: the oracle's composite FIRE on one thread for a
+    bounded number of steps."""
+    L = args.composite_L
+    ks, kb, bmod, smod, dil, gamma = 1.0, 1e-3, 0.5, 0.25, 0.05, 0.05
+    mesh = N.Mesh.lattice(L, 0.6, args.seed)
+    c = N.Composite(mesh, ks, kb, bmod, smod, dil)
+    info = c.info()
+    start = c.positions()
+    prm = N.relax_params(fcut=1e-9, criterion=N.CRIT_RMS, max_steps=2000000, **FIRE)
+    runs = []
+    for _ in range(2):
+        c.set_positions(start)
+        c.affine(info.range, gamma * info.range, 0.0)
+        runs.append(c.relax(prm))
+    r = runs[-1]
+    n, e, t, f = info.n_points, info.n_bonds, info.n_triples, info.n_facets
+    step_bytes = composite_bytes(n, e, t, f)
+    achieved = step_bytes * r["steps"] / (r["ms"] * 1e-3) / 1e9
+    progress(f"composite L={L}: {r['steps']} steps in {r['ms']:.1f} ms")
+    out = {"workload": f"composite L={L}: network p 0.6 on a gel mesh, ks {ks}, kb {kb}, bmod "
+                       f"{bmod}, smod {smod}, dilation {dil}, shear {gamma}, FIRE to "
+                       f"|F|_rms<1e-9, FP64",
+           "points": n, "bonds": e, "triples": t, "facets": f, "pref_d": info.pref_d,
+           "steps": r["steps"], "status": r["status"], "device_ms": r["ms"],
+           "first_relax_ms": runs[0]["ms"], "composites_per_s": 1e3 / r["ms"],
+           "node_steps_per_s": n * r["steps"] / (r["ms"] * 1e-3),
+           "us_per_step": r["ms"] * 1e3 / max(r["steps"], 1),
+           "energies": {"stretch": r["es"], "bend": r["eb"], "gel": r["eg"]},
+           "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "alg_bytes_per_step": step_bytes,
+                        "note": "three launches per FIRE step; achieved = algorithmic bytes "
+                                "of the step x steps / device time"}}
+    if not getattr(args, "no_cpu", False):
+        try:
+            import oracle as O
+            import tempfile
+            with tempfile.TemporaryDirectory() as d:
+                path = os.path.join(d, "c.nmc")
+                mesh.write_nmc(path)
+                o = O.Mesh.read(path)
+                pos, pd = o.setup(ks, kb, bmod, dil)
+                o.affine(pos, gamma * info.range, 0.0, info.range)
+                cap = 500
+                t0 = time.perf_counter()
+                ro = o.fire(ks, kb, smod, bmod, pd, pos, fp=(FIRE["alpha0"], FIRE["f_alpha"],
+                            FIRE["dt_max"], FIRE["f_inc"], FIRE["f_dec"]), nmin=FIRE["nmin"],
+                            mass=FIRE["mass"], fcut=1e-9, max_steps=cap)
+                wall = time.perf_counter() - t0
+            rate = ro["steps"] * n / wall
+            out["cpu_baseline"] = {
+                "value": rate / (n * r["steps"]), "unit": "composites/s", "cores": 1,
+                "kind": "port", "node_steps_per_s": rate,
+                "sample": f"oracle composite FIRE, {ro['steps']} steps on one thread "
+                          f"({wall:.1f} s), divided by the device relax's {r['steps']} steps"}
+            progress(f"composite cpu sample done ({wall:.1f} s)")
+        except Exception as ex:  # the oracle is test infrastructure; report absence
+            out["cpu_baseline"] = {"value": None, "error": str(ex)}
+    if args.workload == "composite":
+        out.update({"metric": "relaxed composites/s (network + gel, |F|_rms<1e-9)",
+                    "value": out["composites_per_s"], "unit": "composites/s", "n_gpus": 1,
+                    "higher_is_better": True, "dtype": "f64", "data": "synthetic"})
+    return out
 
 
 def calibrate_copy(mib):

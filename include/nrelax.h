@@ -32,6 +32,8 @@ extern "C" {
 #define NR_E_TOPO 9      /* topology the device layout cannot hold */
 #define NR_E_BATCH 10    /* batch file invalid (reference :297-383) */
 #define NR_E_TIMEOUT 11  /* a device wait exceeded its bound */
+#define NR_E_MESH 12     /* composite: facet or area section invalid
+                            (network_mesh_relax.cu:154-219) */
 
 /* ---- per-relax outcome (nr_report.status) ----------------------------- */
 #define NR_CONVERGED 0
@@ -282,6 +284,94 @@ int nr_autotune(int count, nr_network *const *nets, const double *ks,
  * strain_mode != 0 scales displacements by the y range (the default). */
 int nr_relaxation_batch(const char *batch_file, int strain_mode,
                         const nr_batch_opts *opts);
+
+/* ---- network + FEM-gel composite (src/network_mesh_relax.cu) ----------- */
+
+/* A composite is a bond/bend network on the points of a triangular gel mesh
+ * (.nmc file: header "points edges facets offset", points, edges
+ * "i j wrap", facets "i j k", one area per facet).  The gel is periodic in x
+ * through twin points: each left-edge point with a point one offset to its
+ * right (within 1e-10); twins receive the sum of their forces. */
+typedef struct nr_mesh nr_mesh;
+typedef struct nr_composite nr_composite;
+
+typedef struct {
+    int n_points, n_bonds, n_facets, n_twins;
+    double offset;      /* header offset (the x period) */
+    double minx, maxx, miny, maxy;
+} nr_mesh_info;
+
+/* read_network_mesh (network_mesh_relax.cu:72-230) with twin pairs
+ * (:219-226).  Errors as nr_network_read, NR_E_MESH for the facet and area
+ * sections. */
+int nr_mesh_read(const char *path, nr_mesh **out);
+/* Synthetic composite (no reference counterpart): the reference-mode L x L
+ * lattice of nr_network_lattice on a mesh of the strip 0 <= x <= L, with
+ * mesh-only points on the strip's edges and each boundary half-cell as a
+ * right triangle. */
+int nr_mesh_lattice(int L, double p, uint64_t seed, nr_mesh **out);
+int nr_mesh_write_nmc(const nr_mesh *m, const char *path);
+int nr_mesh_get_info(const nr_mesh *m, nr_mesh_info *info);
+/* Copies (any may be NULL): pos 2 x points, facets 3 x facets, areas,
+ * twins 2 x twin pairs. */
+int nr_mesh_get_arrays(const nr_mesh *m, double *pos, int *facets, double *areas, int *twins);
+void nr_mesh_free(nr_mesh *m);
+
+/* Line 2 of the composite batch file (parse_batch_file :433-440): fiber
+ * stiffnesses, the gel's bulk and shear moduli (gel_energy :674-699:
+ * A [smod (|F|^2/J - 2) + bmod (J - 1 - d)^2] / 2 per facet) and the
+ * pre-swelling dilation. */
+typedef struct {
+    double ks, kb, bmod, smod, dilation;
+} nr_gel_params;
+
+typedef struct {
+    double width;       /* offset x sqrt(1 + dilation), the bonds' x period */
+    double range;       /* (maxy - miny) x sqrt(1 + dilation) */
+    double pref_d;      /* the gel's preferred dilation d (bmod + net_bmod) / bmod */
+    double net_bmod;    /* 2 E_stretch / (area dilation^2) of the dilated network */
+    int n_points, n_bonds, n_triples, n_facets; /* elements on the device */
+} nr_composite_info;
+
+/* The set-up of do_relaxation_run (:1138-1275) on a copy of m, resident on
+ * device `device`: triples (kb > 0) found with the dilated offset on the
+ * undilated points (as the reference does), clamp codes and the facets'
+ * inverse matrices (get_inv_mats :503-531) from the undilated points, the
+ * points dilated by sqrt(1 + dilation) about (min x, min y) (dilate
+ * :796-806) and the preferred dilation (:1256-1275; 0 when dilation is 0,
+ * where the reference divides 0 by 0).  Terms whose modulus is 0 are left
+ * out, as the reference skips their kernels.  FIRE steps run as replays of
+ * a captured HIP graph; NRELAX_GRAPHS=0 in the environment at creation
+ * launches the same kernels directly instead. */
+int nr_composite_create(const nr_mesh *m, const nr_gel_params *g, int device,
+                        nr_composite **out);
+int nr_composite_get_info(const nr_composite *c, nr_composite_info *info);
+/* affine_displace (:783-793): x += xdisp (y - min y) / yrange, y likewise. */
+int nr_composite_affine(nr_composite *c, double yrange, double xdisp, double ydisp);
+/* run_fire_md (:865-1066) from the resident positions: FIRE in FP64 (method
+ * must be NR_METHOD_FIRE; criterion as nr_relax_params).  rep as for
+ * networks (e_stretch, e_bend, rms/max force, steps, status); *e_gel (may be
+ * NULL) the gel energy. */
+int nr_composite_relax(nr_composite *c, const nr_relax_params *p, nr_report *rep,
+                       double *e_gel);
+int nr_composite_get_positions(nr_composite *c, double *pos);
+int nr_composite_set_positions(nr_composite *c, const double *pos);
+/* The clamped force with twins summed (force: NULL or 2 x points) and
+ * {E_stretch, E_bend, E_gel} (energies: NULL or 3) at the resident
+ * positions, by the relax's own kernels. */
+int nr_composite_eval(nr_composite *c, double *force, double *energies);
+/* Device time of the last relax (ms). */
+double nr_composite_last_ms(const nr_composite *c);
+void nr_composite_destroy(nr_composite *c);
+/* relaxation_batch + do_relaxation_run of the composite program
+ * (:1095-1421): the 6-line batch file (line 2 "ks kb bmod [smod
+ * [dilation]]"), every <base>*.nmc, "Preferred dilation: " on stdout, the
+ * compression ramp from its minimum (the height shrinking with each step),
+ * then the shear ramp, and the reference's outputs: <m>_<tag>_log.txt,
+ * <m>_<tag>_edata.txt (compression "%12.0lf\t%12.10lf" + four energies,
+ * shear "%12.8lf\t%12.8lf" + stretch, bend, gel and total "%12.8le") and
+ * <m>_<tag>_<NNN>.pos after each shear step. */
+int nr_mesh_relaxation_batch(const char *batch_file, int strain_mode, int device);
 
 /* ---- serving ------------------------------------------------------------ */
 

@@ -15,9 +15,11 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libnrelax.so")
 CLI_PATH = os.path.join(_HERE, "nrelax")
 LINEAR_CLI_PATH = os.path.join(_HERE, "nrelax_linear")
+MESH_CLI_PATH = os.path.join(_HERE, "nrelax_mesh")
 
 OK = 0
-E_OPEN, E_HEADER, E_POINTS, E_INDEX, E_EDGES, E_ALLOC, E_ARG, E_HIP, E_TOPO, E_BATCH, E_TIMEOUT = range(1, 12)
+(E_OPEN, E_HEADER, E_POINTS, E_INDEX, E_EDGES, E_ALLOC, E_ARG, E_HIP, E_TOPO, E_BATCH, E_TIMEOUT,
+ E_MESH) = range(1, 13)
 CONVERGED, NOT_CONVERGED, NONFINITE, ABORTED, IN_PROGRESS = range(5)
 CRIT_RMS, CRIT_MAX = 0, 1
 METHOD_FIRE, METHOD_CG = 0, 1
@@ -87,6 +89,23 @@ class LinearReport(C.Structure):
                 ("iterations", C.c_int), ("status", C.c_int), ("device_ms", C.c_double)]
 
 
+class MeshInfo(C.Structure):
+    _fields_ = [("n_points", C.c_int), ("n_bonds", C.c_int), ("n_facets", C.c_int),
+                ("n_twins", C.c_int), ("offset", C.c_double), ("minx", C.c_double),
+                ("maxx", C.c_double), ("miny", C.c_double), ("maxy", C.c_double)]
+
+
+class GelParams(C.Structure):
+    _fields_ = [("ks", C.c_double), ("kb", C.c_double), ("bmod", C.c_double),
+                ("smod", C.c_double), ("dilation", C.c_double)]
+
+
+class CompositeInfo(C.Structure):
+    _fields_ = [("width", C.c_double), ("range", C.c_double), ("pref_d", C.c_double),
+                ("net_bmod", C.c_double), ("n_points", C.c_int), ("n_bonds", C.c_int),
+                ("n_triples", C.c_int), ("n_facets", C.c_int)]
+
+
 class NRError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"nrelax error {code}: {msg}")
@@ -151,6 +170,25 @@ def lib():
         L.nr_server_result.argtypes = [vp, C.c_int, P(JobResult), P(Report), dp]
         L.nr_server_stats.argtypes = [vp, P(ServerInfo)]
         L.nr_server_destroy.argtypes = [vp]
+        L.nr_mesh_read.argtypes = [C.c_char_p, P(vp)]
+        L.nr_mesh_lattice.argtypes = [C.c_int, C.c_double, C.c_uint64, P(vp)]
+        L.nr_mesh_write_nmc.argtypes = [vp, C.c_char_p]
+        L.nr_mesh_get_info.argtypes = [vp, P(MeshInfo)]
+        L.nr_mesh_get_arrays.argtypes = [vp, dp, ip, dp, ip]
+        L.nr_mesh_free.argtypes = [vp]
+        L.nr_mesh_free.restype = None
+        L.nr_composite_create.argtypes = [vp, P(GelParams), C.c_int, P(vp)]
+        L.nr_composite_get_info.argtypes = [vp, P(CompositeInfo)]
+        L.nr_composite_affine.argtypes = [vp, C.c_double, C.c_double, C.c_double]
+        L.nr_composite_relax.argtypes = [vp, P(RelaxParams), P(Report), dp]
+        L.nr_composite_get_positions.argtypes = [vp, dp]
+        L.nr_composite_set_positions.argtypes = [vp, dp]
+        L.nr_composite_eval.argtypes = [vp, dp, dp]
+        L.nr_composite_last_ms.argtypes = [vp]
+        L.nr_composite_last_ms.restype = C.c_double
+        L.nr_composite_destroy.argtypes = [vp]
+        L.nr_composite_destroy.restype = None
+        L.nr_mesh_relaxation_batch.argtypes = [C.c_char_p, C.c_int, C.c_int]
         _lib = L
     return _lib
 
@@ -456,3 +494,102 @@ def linear_response(net, ks, kb, shear, compression=0.0, mu=0.0, affine=True, to
     if displacements:
         out["u"] = u
     return out
+
+
+class Mesh:
+    """Host-side network + gel composite (nr_mesh): .nmc reader, lattice."""
+
+    def __init__(self, handle):
+        self.h = C.c_void_p(handle)
+
+    @classmethod
+    def read(cls, path):
+        h = C.c_void_p()
+        _check(lib().nr_mesh_read(os.fsencode(path), C.byref(h)))
+        return cls(h.value)
+
+    @classmethod
+    def lattice(cls, L, p, seed):
+        h = C.c_void_p()
+        _check(lib().nr_mesh_lattice(L, p, seed, C.byref(h)))
+        return cls(h.value)
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.nr_mesh_free(self.h)
+            self.h = None
+
+    def info(self):
+        i = MeshInfo()
+        _check(lib().nr_mesh_get_info(self.h, C.byref(i)))
+        return i
+
+    def arrays(self):
+        i = self.info()
+        pos = np.zeros(2 * i.n_points)
+        fac = np.zeros(3 * max(i.n_facets, 1), np.int32)
+        area = np.zeros(max(i.n_facets, 1))
+        tw = np.zeros(2 * max(i.n_twins, 1), np.int32)
+        ip = C.POINTER(C.c_int)
+        _check(lib().nr_mesh_get_arrays(self.h, _dp(pos), fac.ctypes.data_as(ip), _dp(area),
+                                        tw.ctypes.data_as(ip)))
+        return dict(pos=pos, facets=fac[:3 * i.n_facets].reshape(-1, 3),
+                    areas=area[:i.n_facets], twins=tw[:2 * i.n_twins].reshape(-1, 2))
+
+    def write_nmc(self, path):
+        _check(lib().nr_mesh_write_nmc(self.h, os.fsencode(path)))
+
+
+class Composite:
+    """A composite resident on one GPU (nr_composite): set-up, affine steps,
+    FIRE relax, positions, force and energies."""
+
+    def __init__(self, mesh, ks, kb, bmod, smod=0.0, dilation=0.0, device=0):
+        self.mesh = mesh
+        g = GelParams(ks, kb, bmod, smod, dilation)
+        h = C.c_void_p()
+        _check(lib().nr_composite_create(mesh.h, C.byref(g), device, C.byref(h)))
+        self.h = h
+        self.n = mesh.info().n_points
+
+    def __del__(self):
+        if getattr(self, "h", None) and self.h.value and _lib is not None:
+            _lib.nr_composite_destroy(self.h)
+            self.h = None
+
+    def info(self):
+        i = CompositeInfo()
+        _check(lib().nr_composite_get_info(self.h, C.byref(i)))
+        return i
+
+    def affine(self, yrange, xdisp, ydisp):
+        _check(lib().nr_composite_affine(self.h, yrange, xdisp, ydisp))
+
+    def relax(self, params):
+        rep = Report()
+        eg = C.c_double()
+        _check(lib().nr_composite_relax(self.h, C.byref(params), C.byref(rep), C.byref(eg)))
+        return dict(status=rep.status, steps=rep.steps, rms=rep.rms_force, fmax=rep.max_force,
+                    es=rep.e_stretch, eb=rep.e_bend, eg=eg.value,
+                    ms=lib().nr_composite_last_ms(self.h))
+
+    def positions(self):
+        pos = np.zeros(2 * self.n)
+        _check(lib().nr_composite_get_positions(self.h, _dp(pos)))
+        return pos
+
+    def set_positions(self, pos):
+        pos = np.ascontiguousarray(pos, np.float64)
+        _check(lib().nr_composite_set_positions(self.h, _dp(pos)))
+
+    def eval(self):
+        f = np.zeros(2 * self.n)
+        e = np.zeros(3)
+        _check(lib().nr_composite_eval(self.h, _dp(f), _dp(e)))
+        return f, e
+
+
+def mesh_relaxation_batch(batch_file, strain_mode=True, device=0):
+    """nr_mesh_relaxation_batch: the composite program's whole protocol."""
+    return lib().nr_mesh_relaxation_batch(os.fsencode(batch_file), 1 if strain_mode else 0,
+                                          device)

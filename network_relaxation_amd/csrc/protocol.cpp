@@ -107,12 +107,12 @@ int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err)
 }
 
 // get_file_lists, file_lists.cpp:80-137: files of `dir` matching
-// ^(<base>.*)\.dat; outputs <dir><m>_<tag>_log.txt, _edata.txt and the
+// ^(<base>.*)\.<ext>; outputs <dir><m>_<tag>_log.txt, _edata.txt and the
 // displacement base <dir><m>_<tag>.  The four names of one network are kept
 // together (the reference sorts the four lists independently, which can pair
 // one network's input with another's outputs).
 int file_lists(const std::string &input, const std::string &tag, std::vector<JobFiles> &jobs,
-               std::string &err)
+               std::string &err, const std::string &ext)
 {
     jobs.clear();
     if (input.empty()) { err = "empty base name"; return NR_E_BATCH; }
@@ -148,7 +148,7 @@ int file_lists(const std::string &input, const std::string &tag, std::vector<Job
     if (base == "*") base = "";
     std::regex expr;
     try {
-        expr = std::regex("^(" + base + ".*)\\.dat");
+        expr = std::regex("^(" + base + ".*)\\." + ext);
     } catch (const std::regex_error &) {
         err = "base name is not a valid pattern: " + base;
         return NR_E_BATCH;

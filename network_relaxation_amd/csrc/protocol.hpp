@@ -29,8 +29,9 @@ struct JobFiles {
 };
 
 int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err);
+// ext: the input extension ("dat" networks, "nmc" network + gel composites)
 int file_lists(const std::string &input, const std::string &tag, std::vector<JobFiles> &jobs,
-               std::string &err);
+               std::string &err, const std::string &ext = "dat");
 int dec_digits(int value);
 std::string report_name(const std::string &base, int digits, int count, const std::string &ext);
 int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,

@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by
 bench.py's CPU-baseline leg, as the checker.  The product never imports it.
-The C sources (oracle.c, linear.c, analysis.c, oracle.h) cite the reference
-lines they restate.
+The C sources (oracle.c, linear.c, analysis.c, mesh.c, oracle.h) cite the
+reference lines they restate.
 """
 import ctypes as C
 import os
@@ -50,6 +50,25 @@ class OrBatch(C.Structure):
 class OrCsr(C.Structure):
     _fields_ = [("n", C.c_int), ("nnz", C.c_long), ("rowptr", C.POINTER(C.c_int)),
                 ("col", C.POINTER(C.c_int)), ("val", C.POINTER(C.c_double))]
+
+
+class OrMesh(C.Structure):
+    _fields_ = [
+        ("net", OrNet), ("nf", C.c_int), ("facet", C.POINTER(C.c_int)),
+        ("area", C.POINTER(C.c_double)), ("inv", C.POINTER(C.c_double)),
+        ("np", C.c_int), ("pair", C.POINTER(C.c_int)),
+        ("offset", C.c_double), ("minx", C.c_double), ("maxx", C.c_double),
+    ]
+
+
+class OrMeshBatch(C.Structure):
+    _fields_ = [
+        ("ks", C.c_double), ("kb", C.c_double), ("bmod", C.c_double), ("smod", C.c_double),
+        ("dilation", C.c_double), ("fp", C.c_double * 5),
+        ("nmin", C.c_int), ("mass", C.c_double), ("fcut", C.c_double), ("nmax", C.c_int),
+        ("cmin", C.c_double), ("cmax", C.c_double), ("cinc", C.c_double),
+        ("smin", C.c_double), ("smax", C.c_double), ("sinc", C.c_double),
+    ]
 
 
 _lib = None
@@ -107,6 +126,26 @@ def lib():
         L.or_analyze.argtypes = [P(OrNet), dp, dp, C.c_double, C.c_double, C.c_double,
                                  C.c_double, dp, dp]
         L.or_analyze.restype = None
+        M = P(OrMesh)
+        d = C.c_double
+        L.or_read_mesh.argtypes = [C.c_char_p, M]
+        L.or_mesh_free.argtypes = [M]
+        L.or_mesh_free.restype = None
+        L.or_mesh_inv_mats.argtypes = [M, dp]
+        L.or_mesh_inv_mats.restype = None
+        L.or_mesh_setup.argtypes = [M, d, d, d, d, dp, dp]
+        L.or_gel_energy.argtypes = [M, d, d, d, dp]
+        L.or_gel_energy.restype = d
+        L.or_mesh_forces.argtypes = [M, d, d, d, d, d, dp, dp]
+        L.or_mesh_forces.restype = None
+        L.or_mesh_energies.argtypes = [M, d, d, d, d, d, dp, dp]
+        L.or_mesh_energies.restype = None
+        L.or_mesh_fire.argtypes = [M, d, d, d, d, d, dp, dp, C.c_int, d, d, C.c_int, dp,
+                                   P(C.c_int)]
+        L.or_parse_mesh_batch.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p,
+                                          C.c_int, P(OrMeshBatch)]
+        L.or_run_mesh_protocol.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
+                                           P(OrMeshBatch), C.c_int, dp]
         _lib = L
     return _lib
 
@@ -347,3 +386,139 @@ def dec_digits(v):
 
 def uniform(seed, counter):
     return lib().or_uniform(seed, counter)
+
+
+class Mesh:
+    """Owns one or_mesh: a network on the points of a gel mesh
+    (src/network_mesh_relax.cu)."""
+
+    def __init__(self):
+        self.s = OrMesh()
+        self._owned = False
+
+    @classmethod
+    def read(cls, path):
+        m = cls()
+        rc = lib().or_read_mesh(os.fsencode(path), C.byref(m.s))
+        if rc:
+            raise ValueError(f"or_read_mesh({path}) failed with code {rc}")
+        m._owned = True
+        return m
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and _lib is not None:
+            _lib.or_mesh_free(C.byref(self.s))
+            self._owned = False
+
+    @property
+    def n(self):
+        return self.s.net.n
+
+    @property
+    def e(self):
+        return self.s.net.e
+
+    @property
+    def t(self):
+        return self.s.net.t
+
+    @property
+    def nf(self):
+        return self.s.nf
+
+    def pos(self):
+        return np.ctypeslib.as_array(self.s.net.pos, shape=(2 * self.n,)).copy()
+
+    def bonds(self):
+        return np.ctypeslib.as_array(self.s.net.bond, shape=(2 * max(self.e, 1),))[
+            :2 * self.e].reshape(-1, 2).copy()
+
+    def rest(self):
+        return np.ctypeslib.as_array(self.s.net.rest, shape=(max(self.e, 1),))[:self.e].copy()
+
+    def tris(self):
+        if self.t == 0:
+            return np.zeros((0, 3), np.int32)
+        return np.ctypeslib.as_array(self.s.net.tri, shape=(3 * self.t,)).reshape(-1, 3).copy()
+
+    def tri_wraps(self):
+        if self.t == 0:
+            return np.zeros((0, 4), np.int8)
+        return np.ctypeslib.as_array(self.s.net.twrap, shape=(4 * self.t,)).reshape(-1, 4).copy()
+
+    def fcode(self):
+        return np.ctypeslib.as_array(self.s.net.fcode, shape=(self.n,)).copy()
+
+    def facets(self):
+        return np.ctypeslib.as_array(self.s.facet, shape=(3 * max(self.nf, 1),))[
+            :3 * self.nf].reshape(-1, 3).copy()
+
+    def areas(self):
+        return np.ctypeslib.as_array(self.s.area, shape=(max(self.nf, 1),))[:self.nf].copy()
+
+    def inv(self):
+        return np.ctypeslib.as_array(self.s.inv, shape=(6 * max(self.nf, 1),))[
+            :6 * self.nf].reshape(-1, 6).copy()
+
+    def pairs(self):
+        return np.ctypeslib.as_array(self.s.pair, shape=(2 * max(self.s.np, 1),))[
+            :2 * self.s.np].reshape(-1, 2).copy()
+
+    def setup(self, ks, kb, bmod, dilation):
+        """Dilated start (2n) and preferred dilation; finds the triples (kb > 0)."""
+        pos = np.zeros(2 * self.n)
+        pd = C.c_double()
+        rc = lib().or_mesh_setup(C.byref(self.s), ks, kb, bmod, dilation, _dp(pos),
+                                 C.byref(pd))
+        if rc:
+            raise ValueError(f"or_mesh_setup failed with code {rc}")
+        return pos, pd.value
+
+    def inv_mats(self, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        lib().or_mesh_inv_mats(C.byref(self.s), _dp(pos))
+
+    def gel_energy(self, smod, bmod, pd, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        return lib().or_gel_energy(C.byref(self.s), smod, bmod, pd, _dp(pos))
+
+    def forces(self, ks, kb, smod, bmod, pd, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        f = np.zeros(2 * self.n)
+        lib().or_mesh_forces(C.byref(self.s), ks, kb, smod, bmod, pd, _dp(pos), _dp(f))
+        return f
+
+    def energies(self, ks, kb, smod, bmod, pd, pos):
+        pos = np.ascontiguousarray(pos, dtype=np.float64)
+        e = np.zeros(3)
+        lib().or_mesh_energies(C.byref(self.s), ks, kb, smod, bmod, pd, _dp(pos), _dp(e))
+        return e
+
+    def affine(self, pos, xdisp, ydisp, yrange):
+        lib().or_affine(C.byref(self.s.net), _dp(pos), self.s.net.miny, yrange, xdisp, ydisp)
+
+    def fire(self, ks, kb, smod, bmod, pd, pos, fp=(0.1, 0.99, 0.1, 1.1, 0.5), nmin=5,
+             mass=1.0, fcut=1e-10, max_steps=1000000):
+        fpa = (C.c_double * 5)(*fp)
+        rep = np.zeros(5)
+        steps = C.c_int()
+        rc = lib().or_mesh_fire(C.byref(self.s), ks, kb, smod, bmod, pd, _dp(pos), fpa, nmin,
+                                mass, fcut, max_steps, _dp(rep), C.byref(steps))
+        return dict(rc=rc, rms=rep[0], es=rep[1], eb=rep[2], eg=rep[3], fmax=rep[4],
+                    steps=steps.value)
+
+
+def parse_mesh_batch(path):
+    b = OrMeshBatch()
+    base = C.create_string_buffer(4096)
+    tag = C.create_string_buffer(4096)
+    rc = lib().or_parse_mesh_batch(os.fsencode(path), base, 4096, tag, 4096, C.byref(b))
+    return rc, base.value.decode(), tag.value.decode(), b
+
+
+def run_mesh_protocol(net_file, log_name, edata_name, disp_base, batch, strain_mode=True):
+    pd = C.c_double()
+    rc = lib().or_run_mesh_protocol(os.fsencode(net_file), os.fsencode(log_name),
+                                    os.fsencode(edata_name), os.fsencode(disp_base),
+                                    C.byref(batch), 1 if strain_mode else 0, C.byref(pd))
+    return rc, pd.value

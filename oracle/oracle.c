@@ -5,6 +5,7 @@
  * reference's functions.
  */
 #include "oracle.h"
+#include "internal.h"
 
 #include <float.h>
 #include <math.h>
@@ -13,18 +14,11 @@
 #include <string.h>
 #include <time.h>
 
-#define OR_FLT_TOL 1e-10
-
 /* ------------------------------------------------------------------------ */
 /* std::getline emulation: returns the line without '\n'; *eof is set when   */
 /* end of file was reached while reading (no terminating newline).           */
 /* ------------------------------------------------------------------------ */
-typedef struct {
-    char *buf;
-    size_t cap;
-} or_line;
-
-static int or_getline(FILE *fp, or_line *ln, int *eof)
+int or_getline(FILE *fp, or_line *ln, int *eof)
 {
     size_t len = 0;
     int c;
@@ -62,7 +56,7 @@ void or_free(or_net *net)
  * large_strain_relax.cu:97-99, dx = x_j + W*w - x_i, |r| = sqrt(dx*dx+dy*dy).
  * The y image term is the Lees-Edwards extension; it is +0.0 in reference mode,
  * which leaves the reference's value unchanged. */
-static double or_rest_length(const double *pos, double W, double H, int i,
+double or_rest_length(const double *pos, double W, double H, int i,
                              int j, int wx, int wy)
 {
     double dx = pos[2 * j] + W * wx - pos[2 * i];
@@ -521,12 +515,11 @@ void or_affine(const or_net *net, double *pos, double miny, double yrange,
     }
 }
 
-static void or_norms(const or_net *net, const double *f, double *ff,
-                     double *fmax2)
+static void or_norms(int n, const double *f, double *ff, double *fmax2)
 {
     int i;
     double s = 0.0, m = 0.0;
-    for (i = 0; i < net->n; i++) {
+    for (i = 0; i < n; i++) {
         double a = f[2 * i], b = f[2 * i + 1];
         double q = fma(a, a, b * b);
         s += q;
@@ -542,19 +535,42 @@ static void or_norms(const or_net *net, const double *f, double *ff,
  * P <= 0), then the convergence test on the force of this step.  The RMS
  * criterion is the reference's |F|/sqrt(2N) (:716); the max criterion
  * (extension) is the largest node force magnitude. */
+typedef struct {
+    const or_net *net;
+    double ks, kb;
+} or_net_field;
+
+static void or_net_force(const void *ctx, const double *pos, double *force)
+{
+    const or_net_field *c = (const or_net_field *)ctx;
+    or_forces(c->net, c->ks, c->kb, pos, force);
+}
+
 int or_fire(const or_net *net, double ks, double kb, double *pos,
             const double fp[5], int nmin, double mass, double fcut,
             int max_steps, int criterion, double report[4], int *steps)
 {
-    int ndof = 2 * net->n, i, since = 0, step = 0;
+    or_net_field c = {net, ks, kb};
+    int rc = or_fire_core(net->n, or_net_force, &c, pos, fp, nmin, mass, fcut, max_steps,
+                          criterion, &report[0], &report[3], steps);
+    if (rc == OR_E_ALLOC) return rc;
+    or_energies(net, ks, kb, pos, &report[1], &report[2]);
+    return rc;
+}
+
+int or_fire_core(int n, or_force_fn fn, const void *ctx, double *pos, const double fp[5],
+                 int nmin, double mass, double fcut, int max_steps, int criterion,
+                 double *rms, double *fmax, int *steps)
+{
+    int ndof = 2 * n, i, since = 0, step = 0;
     double a0 = fp[0], fa = fp[1], dtmax = fp[2], finc = fp[3], fdec = fp[4];
     double alpha = a0, dt = dtmax, dtm = 0.5 * dt / mass, dtsq = 0.5 * dt * dt / mass;
     double sqrt_dof = sqrt((double)ndof), ff, fmax2, fmag;
     double *v = (double *)calloc((size_t)ndof, sizeof(double));
     double *f = (double *)malloc(sizeof(double) * (size_t)ndof);
     if (!v || !f) { free(v); free(f); return OR_E_ALLOC; }
-    or_forces(net, ks, kb, pos, f);
-    or_norms(net, f, &ff, &fmax2);
+    fn(ctx, pos, f);
+    or_norms(n, f, &ff, &fmax2);
     fmag = sqrt(ff);
     while (step < max_steps) {
         double P = 0.0, vv = 0.0, vm, c;
@@ -564,7 +580,7 @@ int or_fire(const or_net *net, double ks, double kb, double *pos,
             pos[i] = fma(dtsq, f[i], pos[i]);
             v[i] = fma(dtm, f[i], v[i]);
         }
-        or_forces(net, ks, kb, pos, f);
+        fn(ctx, pos, f);
         for (i = 0; i < ndof; i++) {
             v[i] = fma(dtm, f[i], v[i]);
         }
@@ -572,7 +588,7 @@ int or_fire(const or_net *net, double ks, double kb, double *pos,
             P = fma(v[i], f[i], P);
             vv = fma(v[i], v[i], vv);
         }
-        or_norms(net, f, &ff, &fmax2);
+        or_norms(n, f, &ff, &fmax2);
         vm = sqrt(vv);
         fmag = sqrt(ff);
         c = alpha * vm / fmag;
@@ -600,9 +616,8 @@ int or_fire(const or_net *net, double ks, double kb, double *pos,
             break;
         if (!isfinite(ff)) break;
     }
-    report[0] = fmag / sqrt_dof;
-    report[3] = sqrt(fmax2);
-    or_energies(net, ks, kb, pos, &report[1], &report[2]);
+    *rms = fmag / sqrt_dof;
+    *fmax = sqrt(fmax2);
     *steps = step;
     free(v);
     free(f);
@@ -627,7 +642,7 @@ int or_cg(const or_net *net, double ks, double kb, double *pos, double fcut,
     double *ft = (double *)malloc(sizeof(double) * (size_t)ndof);
     if (!f || !d || !xt || !ft) { rc = OR_E_ALLOC; goto done; }
     or_forces(net, ks, kb, pos, f);
-    or_norms(net, f, &ff, &fmax2);
+    or_norms(net->n, f, &ff, &fmax2);
     gg = ff;
     memcpy(d, f, sizeof(double) * (size_t)ndof);
     while (it < max_iters) {
@@ -672,7 +687,7 @@ int or_cg(const or_net *net, double ks, double kb, double *pos, double fcut,
         step_len = a * sqrt(dd);
         memcpy(pos, xt, sizeof(double) * (size_t)ndof);
         memcpy(f, ft, sizeof(double) * (size_t)ndof);
-        or_norms(net, f, &ff, &fmax2);
+        or_norms(net->n, f, &ff, &fmax2);
         ggn = ff;
         for (i = 0; i < ndof; i++) d[i] = fma(ggn / gg, d[i], f[i]);
         gg = ggn;

@@ -18,7 +18,8 @@
  * (stiffness assembly :286-428, reduced system :560-605, solve and
  * energies :49-96, :676-881); in analysis.c, the reductions of
  * src/na_calc.cpp:308-326, src/strain_calc.cpp:96-120 and
- * src/axial_contribution.cpp:173-190.
+ * src/axial_contribution.cpp:173-190; in mesh.c, the network + FEM-gel
+ * composite of src/network_mesh_relax.cu.
  * plus the extensions the product also carries (Lees-Edwards box, max-force
  * criterion, per-fiber bending stiffness, nonlinear CG).
  *
@@ -64,6 +65,7 @@ typedef struct {
 #define OR_E_EDGES 5
 #define OR_E_ALLOC 6
 #define OR_E_ARG 7
+#define OR_E_FACETS 12  /* composite: facet or area section invalid */
 
 int or_read_network(const char *path, or_net *net);
 int or_net_from_arrays(int n, int e, double width, double height,
@@ -145,6 +147,51 @@ void or_linear_energies(const or_net *net, double ks, double kb, double mu, cons
  * energy, mean bond strain, max |bond strain|}; strain: NULL or e values. */
 void or_analyze(const or_net *net, const double *x0, const double *x, double shift,
                 double shear, double axial, double k, double *strain, double out[4]);
+
+/* Network + FEM-gel composite (oracle/mesh.c, src/network_mesh_relax.cu).
+ * net holds the points (all of them), bonds, triples and clamp codes; its
+ * width is the box offset, grown by sqrt(1 + dilation) at set-up. */
+typedef struct {
+    or_net net;
+    int nf;              /* facets */
+    int *facet;          /* 3 nf point indices */
+    double *area;        /* nf, as read */
+    double *inv;         /* 6 nf inverse matrices (undilated geometry) */
+    int np;              /* twin pairs */
+    int *pair;           /* 2 np: left-edge point, its twin one offset right */
+    double offset;       /* header offset as read */
+    double minx, maxx;   /* extreme x of the input geometry */
+} or_mesh;
+
+typedef struct {
+    double ks, kb, bmod, smod, dilation;
+    double fp[5];
+    int nmin;
+    double mass, fcut;
+    int nmax;
+    double cmin, cmax, cinc;
+    double smin, smax, sinc;
+} or_mesh_batch;
+
+int or_read_mesh(const char *path, or_mesh *m);
+void or_mesh_free(or_mesh *m);
+void or_mesh_inv_mats(or_mesh *m, const double *pos);
+int or_mesh_setup(or_mesh *m, double ks, double kb, double bmod, double dilation, double *pos,
+                  double *pref_d);
+double or_gel_energy(const or_mesh *m, double smod, double bmod, double pd, const double *pos);
+void or_mesh_forces(const or_mesh *m, double ks, double kb, double smod, double bmod, double pd,
+                    const double *pos, double *force);
+void or_mesh_energies(const or_mesh *m, double ks, double kb, double smod, double bmod,
+                      double pd, const double *pos, double e[3]);
+/* report: [0] rms |F|, [1] E_stretch, [2] E_bend, [3] E_gel, [4] max |F| */
+int or_mesh_fire(const or_mesh *m, double ks, double kb, double smod, double bmod, double pd,
+                 double *pos, const double fp[5], int nmin, double mass, double fcut,
+                 int max_steps, double report[5], int *steps);
+int or_parse_mesh_batch(const char *path, char *base, int base_cap, char *tag, int tag_cap,
+                        or_mesh_batch *b);
+int or_run_mesh_protocol(const char *net_file, const char *log_name, const char *edata_name,
+                         const char *disp_base, const or_mesh_batch *b, int strain_mode,
+                         double *pref_d_out);
 
 /* Synthetic triangular lattice (spacing 1, odd rows shifted by 1/2), each
  * candidate bond (node*3 + direction) kept when u < p. */
