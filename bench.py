@@ -130,7 +130,10 @@ def main():
         import network_relaxation_amd as N
         leg = {"single": single_leg, "ramp": ramp_leg, "large": large_leg,
                "composite": composite_leg}[args.workload]
-        print(json.dumps(leg(N, args)), flush=True)
+        res = leg(N, args)
+        if args.calibrate_copy:  # after the timed work, for the counter passes
+            res["copy_calibration"] = calibrate_copy(args.calibrate_copy)
+        print(json.dumps(res), flush=True)
         return
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:

@@ -26,6 +26,16 @@ namespace nr {
 namespace {
 
 constexpr int kThreads = 256;
+constexpr int kXcds = 8;  // MI355X: workgroup b is dispatched to XCD b % 8
+
+// Chunk of workgroup b among g: each XCD takes a contiguous range of chunks,
+// so neighbouring points (which share element slots) meet in one XCD's L2.
+// A bijection on [0, g) for any g.
+__device__ inline int xcd_chunk(int b, int g)
+{
+    const int x = b % kXcds, q = b / kXcds, per = g / kXcds, rem = g % kXcds;
+    return x * per + min(x, rem) + q;
+}
 
 __device__ inline double2 ld2(const double *x, int i)
 {
@@ -41,7 +51,8 @@ __global__ __launch_bounds__(kThreads) void mesh_elements(MeshDev m, int par)
 {
     if (m.st[par * kStSlot + kStDone] != 0.0) return;
     const int total = m.nb + m.nt + m.nf;
-    for (int k = blockIdx.x * kThreads + threadIdx.x; k < total; k += gridDim.x * kThreads) {
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    for (int k = chunk * kThreads + threadIdx.x; k < total; k += gridDim.x * kThreads) {
         if (k < m.nb) {  // axial_forces :534-562
             const int2 ij = m.bond[k];
             const double2 ro = m.bond_geo[k];  // (rest length, x image offset)
@@ -95,25 +106,27 @@ __device__ inline void add_entry(double &sx, double &sy, int e, double2 s)
 }
 
 // Sum of the slots that name point i, in entry order; entries are
-// (slot << 1) | negate.  Four slot loads are issued before their sums so
-// that each thread keeps several in flight.
+// (slot << 1) | negate.  A point has about 14 entries (at most about 21 on a
+// triangular mesh): all of a batch's entry loads, then all its slot loads,
+// are issued before the first sum, so a point costs two memory round trips
+// per batch instead of two per entry.
+constexpr int kEntBatch = 16;
+
 __device__ inline double2 point_sum(const MeshDev &m, int i)
 {
     double sx = 0.0, sy = 0.0;
     const int lo = m.row[i], hi = m.row[i + 1];
-    int k = lo;
-    for (; k + 4 <= hi; k += 4) {
-        const int e0 = m.ent[k], e1 = m.ent[k + 1], e2 = m.ent[k + 2], e3 = m.ent[k + 3];
-        const double2 s0 = m.slot[e0 >> 1], s1 = m.slot[e1 >> 1];
-        const double2 s2 = m.slot[e2 >> 1], s3 = m.slot[e3 >> 1];
-        add_entry(sx, sy, e0, s0);
-        add_entry(sx, sy, e1, s1);
-        add_entry(sx, sy, e2, s2);
-        add_entry(sx, sy, e3, s3);
-    }
-    for (; k < hi; k++) {
-        const int e = m.ent[k];
-        add_entry(sx, sy, e, m.slot[e >> 1]);
+    for (int k = lo; k < hi; k += kEntBatch) {
+        int e[kEntBatch];
+        double2 v[kEntBatch];
+#pragma unroll
+        for (int j = 0; j < kEntBatch; j++) e[j] = k + j < hi ? m.ent[k + j] : -1;
+#pragma unroll
+        for (int j = 0; j < kEntBatch; j++)
+            v[j] = e[j] >= 0 ? m.slot[e[j] >> 1] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int j = 0; j < kEntBatch; j++)
+            if (e[j] >= 0) add_entry(sx, sy, e[j], v[j]);
     }
     return make_double2(sx, sy);
 }
@@ -154,7 +167,8 @@ __global__ __launch_bounds__(kThreads) void mesh_gather(MeshDev m, int par, int 
     if (st[kStDone] != 0.0) return;
     const double dtm = 0.5 * st[kStDt] / m.mass;
     double P = 0.0, vv = 0.0, ff = 0.0, fm = 0.0;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < m.n; i += gridDim.x * kThreads) {
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    for (int i = chunk * kThreads + threadIdx.x; i < m.n; i += gridDim.x * kThreads) {
         const unsigned char c = m.fcode[i];
         const double2 si = point_sum(m, i);
         double fx = (c & 1) ? si.x : 0.0, fy = (c & 2) ? si.y : 0.0;
@@ -265,7 +279,8 @@ __global__ __launch_bounds__(kThreads) void mesh_fire(MeshDev m, int par, int st
         status = kMeshNotConverged;
     }
     const double dtm = 0.5 * dt / m.mass, dtsq = 0.5 * dt * dt / m.mass;
-    for (int i = blockIdx.x * kThreads + threadIdx.x; i < m.n; i += gridDim.x * kThreads) {
+    const int chunk = xcd_chunk(blockIdx.x, gridDim.x);
+    for (int i = chunk * kThreads + threadIdx.x; i < m.n; i += gridDim.x * kThreads) {
         double2 v = ld2(m.v, i);
         const double2 f = ld2(m.f, i);
         if (zero) {
