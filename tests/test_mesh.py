@@ -294,3 +294,48 @@ def test_mesh_protocol_outputs(tmp_path):
     for k in (1, 2):
         assert os.path.exists(f"{d}/cmp_t_{k}.pos")
     assert not os.path.exists(f"{d}/cmp_t_3.pos")
+
+
+def test_product_reader_agrees_with_oracle(tmp_path):
+    """The product's .nmc reader (nr_mesh_read) gives the oracle's points,
+    facets, areas and twins on the hand-written square, and rejects each bad
+    file the oracle rejects, with the reader's codes."""
+    good = ("5 1 2 1.0\n0 0\n1 0\n0 1\n1 1\n0 0\n0 1 0\n0 1 2\n1 3 2\n0.5\n0.5\n")
+    p = os.path.join(str(tmp_path), "sq.nmc")
+    _write(p, good)
+    a = N.Mesh.read(p).arrays()
+    o = O.Mesh.read(p)
+    np.testing.assert_array_equal(a["pos"], o.pos())
+    np.testing.assert_array_equal(a["facets"], o.facets())
+    np.testing.assert_array_equal(a["areas"], o.areas())
+    np.testing.assert_array_equal(a["twins"], o.pairs())
+    bad = {
+        "header": ("5 1 2\n", N.E_HEADER),
+        "points": ("5 1 2 1.0\n0 0\n1 0\n", N.E_POINTS),
+        "edges": ("3 2 1 1.0\n0 0\n1 0\n0 1\n0 1 0\n", N.E_EDGES),
+        "facets_missing": ("3 1 2 1.0\n0 0\n1 0\n0 1\n0 1 0\n0 1 2\n", N.E_MESH),
+        "index": ("3 1 1 1.0\n0 0\n1 0\n0 1\n0 1 0\n0 1 7\n0.5\n", N.E_INDEX),
+        "area": ("3 1 1 1.0\n0 0\n1 0\n0 1\n0 1 0\n0 1 2\n-0.5\n", N.E_MESH),
+        "areas_missing": ("3 1 2 1.0\n0 0\n1 0\n0 1\n0 1 0\n0 1 2\n0 2 1\n0.5\n", N.E_MESH),
+    }
+    for k, (text, code) in bad.items():
+        q = os.path.join(str(tmp_path), f"{k}.nmc")
+        _write(q, text)
+        with pytest.raises(N.NRError) as e:
+            N.Mesh.read(q)
+        assert e.value.code == code, k
+        with pytest.raises(ValueError):
+            O.Mesh.read(q)
+
+
+def test_composite_argument_errors(tmp_path):
+    """Set-up rejects negative moduli or dilation before any device work."""
+    m = N.Mesh.lattice(6, 0.7, 1)
+    for args in [(-1.0, 0.0, 1.0, 0.0, 0.0), (1.0, -1.0, 1.0, 0.0, 0.0),
+                 (1.0, 0.0, -1.0, 0.0, 0.0), (1.0, 0.0, 1.0, 0.0, -0.1),
+                 (1.0, 0.0, 1.0, -0.5, 0.0)]:
+        with pytest.raises(N.NRError) as e:
+            N.Composite(m, *args)
+        assert e.value.code == N.E_ARG
+    with pytest.raises(N.NRError):
+        N.Mesh.lattice(2, 0.5, 1)
