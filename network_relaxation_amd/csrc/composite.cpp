@@ -12,6 +12,7 @@
 // graph launches).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -51,7 +52,7 @@ struct nr_composite {
     nr::DevBuf<double2> slot, bond_geo, tri_off;
     nr::DevBuf<int2> bond;
     nr::DevBuf<int4> tri, facet;
-    nr::DevBuf<int> row, ent, twin;
+    nr::DevBuf<int> ent, cnt, twin;
     nr::DevBuf<unsigned char> fcode;
     nr::MeshDev md{};
     hipGraph_t graph = nullptr;
@@ -75,39 +76,52 @@ struct nr_composite {
 namespace nr {
 namespace {
 
-// Slot entries per point, (slot << 1) | negate, in element order: bond b is
-// slot b (+ on its first point, - on its second); triple t is slots
-// nb + 2t (end a) and nb + 2t + 1 (end b), both negated on the hinge; facet
-// f is slots nb + 2nt + 3f + k on its corner k.
-void build_gather(int n, const std::vector<int> &bonds, int nb, const std::vector<int> &tri, int nt,
-                  const std::vector<int> &facets, int nf, std::vector<int> &row,
-                  std::vector<int> &ent)
+// Slot entries per point, (slot << 1) | negate, in element order, in three
+// sections: bonds (bond b is slot b, + on its first point, - on its second),
+// triples (triple t is slots nb + t for end a and nb + nt + t for end b,
+// both negated on the hinge) and facets (facet f is slot
+// nb + 2nt + k nf + f on its corner k).  Each section is column-major with the section's largest
+// per-point count as its width: entry k of point i at sec[s] + k * n + i.
+// cnt packs the three counts of a point in 8 bits each.
+int build_gather(int n, const std::vector<int> &bonds, int nb, const std::vector<int> &tri, int nt,
+                 const std::vector<int> &facets, int nf, std::vector<int> &ent,
+                 std::vector<int> &cnt, long sec[3])
 {
-    row.assign((size_t)n + 1, 0);
-    for (int b = 0; b < 2 * nb; b++) row[(size_t)bonds[b] + 1]++;
-    for (int t = 0; t < nt; t++) {
-        row[(size_t)tri[3 * t] + 1] += 2;
-        row[(size_t)tri[3 * t + 1] + 1]++;
-        row[(size_t)tri[3 * t + 2] + 1]++;
-    }
-    for (int f = 0; f < 3 * nf; f++) row[(size_t)facets[f] + 1]++;
-    for (int i = 0; i < n; i++) row[(size_t)i + 1] += row[(size_t)i];
-    ent.assign((size_t)row[(size_t)n], 0);
-    std::vector<int> fill(row.begin(), row.end() - 1);
-    auto put = [&](int point, int e) { ent[(size_t)fill[(size_t)point]++] = e; };
+    std::vector<std::vector<int>> lists(3 * (size_t)n);
+    auto put = [&](int s, int point, int e) { lists[3 * (size_t)point + s].push_back(e); };
     for (int b = 0; b < nb; b++) {
-        put(bonds[2 * b], b << 1);
-        put(bonds[2 * b + 1], (b << 1) | 1);
+        put(0, bonds[2 * b], b << 1);
+        put(0, bonds[2 * b + 1], (b << 1) | 1);
     }
     for (int t = 0; t < nt; t++) {
-        const int sa = nb + 2 * t, sb = sa + 1;
-        put(tri[3 * t], (sa << 1) | 1);
-        put(tri[3 * t], (sb << 1) | 1);
-        put(tri[3 * t + 1], sa << 1);
-        put(tri[3 * t + 2], sb << 1);
+        const int sa = nb + t, sb = nb + nt + t;
+        put(1, tri[3 * t], (sa << 1) | 1);
+        put(1, tri[3 * t], (sb << 1) | 1);
+        put(1, tri[3 * t + 1], sa << 1);
+        put(1, tri[3 * t + 2], sb << 1);
     }
     for (int f = 0; f < nf; f++)
-        for (int k = 0; k < 3; k++) put(facets[3 * f + k], (nb + 2 * nt + 3 * f + k) << 1);
+        for (int k = 0; k < 3; k++) put(2, facets[3 * f + k], (nb + 2 * nt + k * nf + f) << 1);
+    size_t width[3] = {0, 0, 0};
+    for (int i = 0; i < n; i++)
+        for (int s = 0; s < 3; s++)
+            width[s] = std::max(width[s], lists[3 * (size_t)i + s].size());
+    if (width[0] > 255 || width[1] > 255 || width[2] > 255)
+        return fail(NR_E_TOPO, "a point has more than 255 entries of one element type");
+    sec[0] = 0;
+    sec[1] = (long)(width[0] * (size_t)n);
+    sec[2] = sec[1] + (long)(width[1] * (size_t)n);
+    ent.assign((size_t)sec[2] + width[2] * (size_t)n + 1, 0);
+    cnt.assign((size_t)n, 0);
+    for (int i = 0; i < n; i++) {
+        for (int s = 0; s < 3; s++) {
+            const std::vector<int> &l = lists[3 * (size_t)i + s];
+            for (size_t k = 0; k < l.size(); k++) ent[(size_t)sec[s] + k * (size_t)n + i] = l[k];
+        }
+        cnt[i] = (int)lists[3 * (size_t)i].size() | (int)lists[3 * (size_t)i + 1].size() << 8 |
+                 (int)lists[3 * (size_t)i + 2].size() << 16;
+    }
+    return NR_OK;
 }
 
 int composite_upload(nr_composite &c)
@@ -119,8 +133,10 @@ int composite_upload(nr_composite &c)
     const int nb = c.gel.ks > 0 ? net.e() : 0;
     const int nt = c.gel.kb > 0 ? net.t() : 0;
     const int nf = c.gel.bmod > 0 ? c.mesh.nf() : 0;
-    std::vector<int> row, ent;
-    build_gather(n, net.bonds, nb, net.tri, nt, c.mesh.facets, nf, row, ent);
+    std::vector<int> ent, cnt;
+    long sec[3];
+    int rc = build_gather(n, net.bonds, nb, net.tri, nt, c.mesh.facets, nf, ent, cnt, sec);
+    if (rc) return rc;
     std::vector<int2> bond((size_t)nb);
     std::vector<double2> geo((size_t)nb);
     for (int b = 0; b < nb; b++) {
@@ -160,8 +176,8 @@ int composite_upload(nr_composite &c)
     HIP_TRY(c.inv.upload(inv));
     HIP_TRY(c.area.upload(area));
     HIP_TRY(c.slot.alloc(nslot));
-    HIP_TRY(c.row.upload(row));
     HIP_TRY(c.ent.upload(ent));
+    HIP_TRY(c.cnt.upload(cnt));
     if (!twin.empty()) HIP_TRY(c.twin.upload(twin));
     HIP_TRY(c.fcode.upload(net.fcode));
     HIP_TRY(c.part.alloc(4 * (size_t)kMeshMaxBlocks));
@@ -188,8 +204,9 @@ int composite_upload(nr_composite &c)
     m.inv = c.inv.p;
     m.area = c.area.p;
     m.slot = c.slot.p;
-    m.row = c.row.p;
     m.ent = c.ent.p;
+    m.cnt = c.cnt.p;
+    for (int k = 0; k < 3; k++) m.sec[k] = sec[k];
     m.twin = twin.empty() ? nullptr : c.twin.p;
     m.fcode = c.fcode.p;
     m.part = c.part.p;

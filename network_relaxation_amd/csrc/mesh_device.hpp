@@ -28,9 +28,15 @@ struct MeshDev {
     const int4 *facet;           // (p1, p2, p3, 0)
     const double *inv;           // 6 per facet
     const double *area;          // per facet
-    double2 *slot;               // nb + 2 nt + 3 nf element forces
-    const int *row;              // n + 1: slot entries of each point
-    const int *ent;              // (slot << 1) | negate, in element order
+    double2 *slot;               // element forces: bonds [0, nb), triple ends a then
+                                 // b (nt each), facet corners 0, 1, 2 (nf each)
+    // Slot entries of each point, (slot << 1) | negate, in element order, in
+    // three sections (bonds, triples, facets), each stored column-major:
+    // entry k of point i in section s at ent[sec[s] + k * n + i], so the
+    // k-th entries of consecutive points are consecutive words.
+    const int *ent;
+    const int *cnt;              // n: entries per section, 8 bits each
+    long sec[3];                 // section offsets in ent
     const int *twin;             // n: twin point or -1; null when twins are not coupled
     const unsigned char *fcode;  // n clamp codes
     double *part;                // 4 per gather workgroup

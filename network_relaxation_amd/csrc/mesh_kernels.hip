@@ -3,8 +3,9 @@
 // composite, every vector and scalar resident on the device.  One FIRE step
 // is three launches:
 //   mesh_elements  one thread per bond, triple and facet; each writes its
-//                  forces into its own slots, in element order (contiguous
-//                  writes, no atomics)
+//                  forces into its own slots (one array per bond, triple end
+//                  and facet corner, in element order: contiguous writes, no
+//                  atomics)
 //   mesh_gather    one thread per point: the clamped sum of the slots that
 //                  name it (its twin's too when the gel couples twins: the
 //                  reference's duplicate_forces :754-779 as a gather), the
@@ -72,8 +73,8 @@ __global__ __launch_bounds__(kThreads) void mesh_elements(MeshDev m, int par)
             const double cr = fma(ax, by, -(ay * bx)), dt = fma(ax, bx, ay * by);
             const double g = m.kb * atan2(-cr, -dt);  // kb (theta - pi)
             const double f2 = g * ia, f3 = g * ib;
-            m.slot[m.nb + 2 * t] = make_double2(f2 * -ay, f2 * ax);
-            m.slot[m.nb + 2 * t + 1] = make_double2(f3 * by, -f3 * bx);
+            m.slot[m.nb + t] = make_double2(f2 * -ay, f2 * ax);
+            m.slot[m.nb + m.nt + t] = make_double2(f3 * by, -f3 * bx);
         } else {  // gel_forces :702-750
             const int f = k - m.nb - m.nt;
             const int4 v = m.facet[f];
@@ -91,10 +92,10 @@ __global__ __launch_bounds__(kThreads) void mesh_elements(MeshDev m, int par)
             const double c2 = (m.bmod * (m.pref_d + 1.0 - J) + m.smod * I1 / (2.0 * J * J)) * A;
             const double d11 = fma(f11, c1, f22 * c2), d12 = fma(f12, c1, -(f21 * c2));
             const double d21 = fma(f21, c1, -(f12 * c2)), d22 = fma(f22, c1, f11 * c2);
-            double2 *o = m.slot + m.nb + 2 * m.nt + 3 * (size_t)f;
+            double2 *o = m.slot + m.nb + 2 * (size_t)m.nt + f;  // one array per corner
             o[0] = make_double2(fma(d12, m1, d11 * m0), fma(d22, m1, d21 * m0));
-            o[1] = make_double2(fma(d12, m3, d11 * m2), fma(d22, m3, d21 * m2));
-            o[2] = make_double2(fma(d12, m5, d11 * m4), fma(d22, m5, d21 * m4));
+            o[m.nf] = make_double2(fma(d12, m3, d11 * m2), fma(d22, m3, d21 * m2));
+            o[2 * (size_t)m.nf] = make_double2(fma(d12, m5, d11 * m4), fma(d22, m5, d21 * m4));
         }
     }
 }
@@ -105,22 +106,22 @@ __device__ inline void add_entry(double &sx, double &sy, int e, double2 s)
     sy += (e & 1) ? -s.y : s.y;
 }
 
-// Sum of the slots that name point i, in entry order; entries are
-// (slot << 1) | negate.  A point has about 14 entries (at most about 21 on a
-// triangular mesh): all of a batch's entry loads, then all its slot loads,
-// are issued before the first sum, so a point costs two memory round trips
-// per batch instead of two per entry.
-constexpr int kEntBatch = 16;
+// Sum of the slots that name point i, in entry order (bonds, then triples,
+// then facets); entries are (slot << 1) | negate.  Each section's entries
+// are loaded a batch at a time before their slots, and the slot loads before
+// the first sum, so a point costs a few memory round trips, not two per
+// entry; column-major sections make each load coalesced across the wave.
+constexpr int kEntBatch = 8;
 
-__device__ inline double2 point_sum(const MeshDev &m, int i)
+__device__ inline void sum_section(const MeshDev &m, long base, int count, int i, double &sx,
+                                   double &sy)
 {
-    double sx = 0.0, sy = 0.0;
-    const int lo = m.row[i], hi = m.row[i + 1];
-    for (int k = lo; k < hi; k += kEntBatch) {
+    for (int k = 0; k < count; k += kEntBatch) {
         int e[kEntBatch];
         double2 v[kEntBatch];
 #pragma unroll
-        for (int j = 0; j < kEntBatch; j++) e[j] = k + j < hi ? m.ent[k + j] : -1;
+        for (int j = 0; j < kEntBatch; j++)
+            e[j] = k + j < count ? m.ent[base + (long)(k + j) * m.n + i] : -1;
 #pragma unroll
         for (int j = 0; j < kEntBatch; j++)
             v[j] = e[j] >= 0 ? m.slot[e[j] >> 1] : make_double2(0.0, 0.0);
@@ -128,6 +129,15 @@ __device__ inline double2 point_sum(const MeshDev &m, int i)
         for (int j = 0; j < kEntBatch; j++)
             if (e[j] >= 0) add_entry(sx, sy, e[j], v[j]);
     }
+}
+
+__device__ inline double2 point_sum(const MeshDev &m, int i)
+{
+    double sx = 0.0, sy = 0.0;
+    const int c = m.cnt[i];
+    sum_section(m, m.sec[0], c & 255, i, sx, sy);
+    sum_section(m, m.sec[1], (c >> 8) & 255, i, sx, sy);
+    sum_section(m, m.sec[2], (c >> 16) & 255, i, sx, sy);
     return make_double2(sx, sy);
 }
 
