@@ -373,6 +373,27 @@ void nr_composite_destroy(nr_composite *c);
  * <m>_<tag>_<NNN>.pos after each shear step. */
 int nr_mesh_relaxation_batch(const char *batch_file, int strain_mode, int device);
 
+/* ---- the XML job format (src/large_strain_types.hpp) ------------------- */
+
+/* The reference declares an XML relaxation job (element names, children and
+ * attributes :36-50, RelaxScheme / Run / RelaxBatch :99-168, the description
+ * :226-250) but never parses it (parse_relaxation_batch :266-273 is empty).
+ * <rbatch> holds <name input tag>, <mech_params stretch bend>, <rscheme
+ * type="FIRE|ConjGrad" pars="..."> and one or more <run> with <axial min max
+ * inc>, <shear min max inc> and <report edata log disp>.  FIRE pars are the
+ * batch file's lines 3 and 4 (alpha0 f_alpha dt_max f_inc f_dec N_min mass
+ * fcut max_steps); ConjGrad pars "fcut max_iters" or the same nine.  Each
+ * <base>*.dat runs its runs in order on its resident state, as the batch
+ * protocol (nr_relaxation_batch) does one ramp pair; one run writes the batch
+ * protocol's files, several write run k's as <m>_<tag>_run<k>_... ; a false
+ * report flag skips that file.  NR_E_BATCH for a file that breaks the
+ * format (nr_last_error says where). */
+int nr_xml_batch(const char *job_file, int strain_mode, const nr_batch_opts *opts);
+/* Parses and checks a job file and lists what nr_xml_batch would do (the
+ * job's settings, then one line per network and run with its output names)
+ * into plan (NUL-terminated, at most cap bytes).  No GPU work. */
+int nr_xml_job_plan(const char *job_file, char *plan, int cap);
+
 /* ---- serving ------------------------------------------------------------ */
 
 /* A request loop with continuous batching on one GPU; no counterpart in the

@@ -189,6 +189,8 @@ def lib():
         L.nr_composite_destroy.argtypes = [vp]
         L.nr_composite_destroy.restype = None
         L.nr_mesh_relaxation_batch.argtypes = [C.c_char_p, C.c_int, C.c_int]
+        L.nr_xml_batch.argtypes = [C.c_char_p, C.c_int, P(BatchOpts)]
+        L.nr_xml_job_plan.argtypes = [C.c_char_p, C.c_char_p, C.c_int]
         _lib = L
     return _lib
 
@@ -593,3 +595,17 @@ def mesh_relaxation_batch(batch_file, strain_mode=True, device=0):
     """nr_mesh_relaxation_batch: the composite program's whole protocol."""
     return lib().nr_mesh_relaxation_batch(os.fsencode(batch_file), 1 if strain_mode else 0,
                                           device)
+
+
+def xml_job_plan(job_file):
+    """nr_xml_job_plan: parse and check an XML job file; the plan as text
+    (no GPU work).  Raises NRError for a file that breaks the format."""
+    buf = C.create_string_buffer(1 << 16)
+    _check(lib().nr_xml_job_plan(os.fsencode(job_file), buf, len(buf)))
+    return buf.value.decode()
+
+
+def xml_batch(job_file, strain_mode=True, precision=PREC_FP64):
+    """nr_xml_batch: run an XML job (several runs per network)."""
+    opts = BatchOpts(precision, 0, 0, 0, 0, 0, 0.0)
+    return lib().nr_xml_batch(os.fsencode(job_file), 1 if strain_mode else 0, C.byref(opts))

@@ -212,33 +212,34 @@ void log_result(FILE *lf, const nr_report &r, double fcut, int criterion)
 
 }  // namespace
 
-// do_relaxation_run :792-1010.
-int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
-                 const nr_batch_opts *opts, std::string &err)
+// The set-up of do_relaxation_run (:816-902): read, triples, device arrays.
+int proto_open(const JobFiles &jf, const BatchSpec &b, const nr_batch_opts *opts, ProtoNet &pn,
+               std::string &err)
 {
-    std::clock_t ct = std::clock();
-    nr_network *net = nullptr;
-    int rc = nr_network_read(jf.dat.c_str(), &net);
+    int rc = nr_network_read(jf.dat.c_str(), &pn.n);
     if (rc) {
         err = nr_last_error();
         return rc;
     }
-    struct Guard {
-        nr_network *n = nullptr;
-        nr_batch *b = nullptr;
-        ~Guard()
-        {
-            nr_batch_destroy(b);
-            nr_network_free(n);
-        }
-    } g;
-    g.n = net;
     if (b.kb > 0) {
-        nr_network_build_triples(net);
-        if (b.kappa_hi > 0) nr_network_set_fiber_kappa(net, b.kappa_lo, b.kappa_hi, b.kappa_seed);
+        nr_network_build_triples(pn.n);
+        if (b.kappa_hi > 0) nr_network_set_fiber_kappa(pn.n, b.kappa_lo, b.kappa_hi, b.kappa_seed);
     }
-    nr_network_info info;
-    nr_network_get_info(net, &info);
+    nr_network_get_info(pn.n, &pn.info);
+    rc = nr_batch_create(1, &pn.n, &b.ks, &b.kb, opts, &pn.b);
+    if (rc) err = nr_last_error();
+    return rc;
+}
+
+// The compression then shear ramps of do_relaxation_run (:917-1000) from the
+// network's resident state, each increment an affine step of `inc` (the first
+// one included) and a relax; the log, edata and .pos outputs, each when its
+// flag is set.
+int proto_ramps(ProtoNet &pn, const JobFiles &jf, const BatchSpec &b, bool strain_mode,
+                unsigned report, std::string &err)
+{
+    std::clock_t ct = std::clock();
+    const nr_network_info &info = pn.info;
     const double range = info.height > 0 ? info.height : info.maxy - info.miny;
     const double smult = strain_mode ? range : 1.0;
     if ((b.cmax > 0 && !(b.cinc > 0)) || (b.smax > 0 && !(b.sinc > 0))) {
@@ -250,11 +251,6 @@ int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
         return NR_E_BATCH;
     }
     const int num_digits = dec_digits(1 + (b.sinc > 0 ? (int)((b.smax - b.smin) / b.sinc) : 0));
-    rc = nr_batch_create(1, &net, &b.ks, &b.kb, opts, &g.b);
-    if (rc) {
-        err = nr_last_error();
-        return rc;
-    }
     nr_relax_params rp;
     rp.alpha0 = b.fp[0];
     rp.f_alpha = b.fp[1];
@@ -268,15 +264,16 @@ int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
     rp.criterion = b.criterion;
     rp.method = b.method;
 
-    FILE *lf = std::fopen(jf.log.c_str(), "w");
-    FILE *ef = std::fopen(jf.edata.c_str(), "w");
+    FILE *lf = (report & kReportLog) ? std::fopen(jf.log.c_str(), "w") : nullptr;
+    FILE *ef = (report & kReportEdata) ? std::fopen(jf.edata.c_str(), "w") : nullptr;
     std::vector<double> pos(2 * (size_t)info.n_nodes);
     double mn = b.cmin * smult, mx = b.cmax * smult, inc = b.cinc * smult, cur = 0.0;
     nr_report rep;
+    int rc = 0;
     if (mx > 0) {
         for (cur = mn; cur <= mx + kFltTol; cur += inc) {
-            if ((rc = nr_batch_strain(g.b, 0, 0.0, -inc)) != 0 ||
-                (rc = nr_batch_relax(g.b, &rp, 1, &rep)) != 0) {
+            if ((rc = nr_batch_strain(pn.b, 0, 0.0, -inc)) != 0 ||
+                (rc = nr_batch_relax(pn.b, &rp, 1, &rep)) != 0) {
                 err = nr_last_error();
                 break;
             }
@@ -293,9 +290,10 @@ int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
     int disp_count = 1;
     if (rc == 0 && mx > 0) {
         for (cur = mn; cur <= mx + kFltTol; cur += inc) {
-            if ((rc = nr_batch_strain(g.b, 0, inc, 0.0)) != 0 ||
-                (rc = nr_batch_relax(g.b, &rp, 1, &rep)) != 0 ||
-                (rc = nr_batch_get_positions(g.b, 0, pos.data())) != 0) {
+            if ((rc = nr_batch_strain(pn.b, 0, inc, 0.0)) != 0 ||
+                (rc = nr_batch_relax(pn.b, &rp, 1, &rep)) != 0 ||
+                ((report & kReportDisp) &&
+                 (rc = nr_batch_get_positions(pn.b, 0, pos.data())) != 0)) {
                 err = nr_last_error();
                 break;
             }
@@ -303,6 +301,7 @@ int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
             if (ef)
                 std::fprintf(ef, "%12.10lf\t%12.10lf\t%12.10le\t%12.10le\t%12.10le\n", cur, max_comp,
                              rep.e_stretch, rep.e_bend, rep.e_stretch + rep.e_bend);
+            if (!(report & kReportDisp)) continue;
             std::string name = report_name(jf.disp, num_digits, disp_count++, "pos");
             FILE *pf = std::fopen(name.c_str(), "w");
             if (pf) {
@@ -319,6 +318,16 @@ int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
     }
     if (ef) std::fclose(ef);
     return rc;
+}
+
+// do_relaxation_run :792-1010.
+int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
+                 const nr_batch_opts *opts, std::string &err)
+{
+    ProtoNet pn;
+    int rc = proto_open(jf, b, opts, pn, err);
+    if (rc) return rc;
+    return proto_ramps(pn, jf, b, strain_mode, kReportAll, err);
 }
 
 // parse_batch_file of sparse_relax.cpp (:920-1035).

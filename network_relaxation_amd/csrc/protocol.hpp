@@ -37,6 +37,27 @@ std::string report_name(const std::string &base, int digits, int count, const st
 int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
                  const nr_batch_opts *opts, std::string &err);
 
+// Outputs of a ramp (the XML job format's report flags).
+constexpr unsigned kReportEdata = 1, kReportLog = 2, kReportDisp = 4, kReportAll = 7;
+// One network of a batch, resident on the device for all its ramps.
+struct ProtoNet {
+    nr_network *n = nullptr;
+    nr_batch *b = nullptr;
+    nr_network_info info{};
+    ProtoNet() = default;
+    ProtoNet(const ProtoNet &) = delete;
+    ProtoNet &operator=(const ProtoNet &) = delete;
+    ~ProtoNet()
+    {
+        nr_batch_destroy(b);
+        nr_network_free(n);
+    }
+};
+int proto_open(const JobFiles &jf, const BatchSpec &b, const nr_batch_opts *opts, ProtoNet &pn,
+               std::string &err);
+int proto_ramps(ProtoNet &pn, const JobFiles &jf, const BatchSpec &b, bool strain_mode,
+                unsigned report, std::string &err);
+
 // Batch file of the reference's small-strain program `sp_relax`
 // (sparse_relax.cpp:920-1035): base [tag]; ks kb mu; compression min max
 // inc; shear min max inc.
