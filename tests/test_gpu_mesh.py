@@ -172,3 +172,20 @@ def test_mesh_cli_matches_oracle(tmp_path):
         pg = np.loadtxt(f"{d}/gpu/cmp_t_{k}.pos")
         pc = np.loadtxt(f"{d}/cpu/cmp_t_{k}.pos")
         np.testing.assert_allclose(pg, pc, rtol=0, atol=1e-7)
+
+
+def test_direct_launches_equal_graph_replays(tmp_path, monkeypatch):
+    """NRELAX_GRAPHS=0 at creation launches the same kernels directly: the
+    same steps, energies and positions, bit for bit."""
+    _say("case direct launches vs graph replays")
+    mesh = N.Mesh.lattice(24, 0.65, 5)
+    out = []
+    for env in ("1", "0"):
+        monkeypatch.setenv("NRELAX_GRAPHS", env)
+        c = N.Composite(mesh, 1.0, 1e-3, 0.5, 0.25, 0.05)
+        yr = c.info().range
+        c.affine(yr, 0.03 * yr, 0.0)
+        r = c.relax(N.relax_params(fcut=1e-11, criterion=N.CRIT_RMS))
+        out.append((r["steps"], r["es"], r["eb"], r["eg"], c.positions()))
+    assert out[0][:4] == out[1][:4]
+    np.testing.assert_array_equal(out[0][4], out[1][4])
