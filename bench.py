@@ -492,6 +492,11 @@ def served_leg(N, args):
                                          "n": int(len(errs))}}
 
 
+# HBM bytes per composite FIRE step at L = 512 (FETCH_SIZE x 2 + WRITE_SIZE,
+# KiB x 1024, summed over the step's three kernels: profiles/mesh_kernels_composite512.md)
+COMPOSITE512_TRAFFIC_BYTES = ((34216 * 2 + 40352) + (34864 * 2 + 8201) + (6124 * 2 + 8135)) * 1024
+
+
 def composite_bytes(n, e, t, f):
     """Algorithmic bytes of one composite FIRE step (three launches):
     mesh_elements reads topology + positions and writes slots (bond 72 B,
@@ -539,9 +544,11 @@ def composite_leg(N, args):
            "energies": {"stretch": r["es"], "bend": r["eb"], "gel": r["eg"]},
            "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                        "traffic": COMPOSITE512_TRAFFIC_BYTES if L == 512 else None,
                         "alg_bytes_per_step": step_bytes,
                         "note": "three launches per FIRE step; achieved = algorithmic bytes "
-                                "of the step x steps / device time"}}
+                                "of the step x steps / device time; traffic = counted HBM "
+                                "bytes per step"}}
     if not getattr(args, "no_cpu", False):
         try:
             import oracle as O
