@@ -140,3 +140,39 @@ def test_job_runs_continue_the_batch_protocol(tmp_path):
     for k in (1, 2):
         np.testing.assert_array_equal(np.loadtxt(f"{d}/xml/net0_t_run2_{k}.pos"),
                                       np.loadtxt(f"{d}/bat/net0_t_{k + 3}.pos"))
+
+
+def test_mutated_job_files_fail_cleanly(tmp_path):
+    """Random byte edits, truncations and insertions of a valid job file:
+    the reader returns success or NR_E_BATCH, never anything else (and never
+    crashes the process)."""
+    base = _base(tmp_path)
+    good = (f'<?xml version="1.0"?><rbatch><name input="{base}" tag="t"/>'
+            f'<mech_params stretch="1.0" bend="2e-3"/>'
+            f'<rscheme type="FIRE" pars="{FIRE_PARS}"/>'
+            '<run><axial min="0" max="0.02" inc="0.01"/><shear min="0" max="0.1" inc="0.05"/>'
+            '<report edata="true" log="false" disp="1"/></run></rbatch>')
+    assert N.xml_job_plan(_job(tmp_path, good, "good.xml"))
+    rng = np.random.default_rng(7)
+    alphabet = b'<>/="\' \n!?-abcdefghijklmnopqrstuvwxyz0123456789.&;'
+    for trial in range(400):
+        b = bytearray(good.encode())
+        for _ in range(int(rng.integers(1, 4))):
+            op = int(rng.integers(0, 3))
+            k = int(rng.integers(0, len(b)))
+            if op == 0:
+                b[k] = alphabet[int(rng.integers(0, len(alphabet)))]
+            elif op == 1:
+                del b[k:k + int(rng.integers(1, 8))]
+            else:
+                b[k:k] = bytes(alphabet[int(rng.integers(0, len(alphabet)))]
+                               for _ in range(int(rng.integers(1, 6))))
+        if trial % 50 == 0:
+            b = b[:int(rng.integers(0, len(b)))]
+        p = os.path.join(str(tmp_path), "m.xml")
+        with open(p, "wb") as f:
+            f.write(bytes(b))
+        try:
+            N.xml_job_plan(p)
+        except N.NRError as e:
+            assert e.code == N.E_BATCH, (e.code, bytes(b))
