@@ -176,3 +176,25 @@ def test_mutated_job_files_fail_cleanly(tmp_path):
             N.xml_job_plan(p)
         except N.NRError as e:
             assert e.code == N.E_BATCH, (e.code, bytes(b))
+
+
+@pytest.mark.gpu
+def test_conjgrad_job_equals_the_cg_batch(tmp_path):
+    """rscheme type="ConjGrad" pars="fcut max_iters" runs the batch file's
+    nonlinear CG (line 4 "... cg"): the same energy file line for line."""
+    print("case xml ConjGrad job vs cg batch", flush=True)
+    d = str(tmp_path)
+    os.makedirs(f"{d}/xml")
+    os.makedirs(f"{d}/bat")
+    for sub in ("xml", "bat"):
+        N.Network.lattice(10, 0.7, 4, False).triples().write_dat(f"{d}/{sub}/net0.dat")
+    with open(f"{d}/bat/b.txt", "w") as f:
+        f.write(f"{d}/bat/net t\n1.0 2e-3\n0.1 0.99 0.1 1.1 0.5\n5 1.0 1e-10 100000 rms cg\n"
+                "0 0 0\n0.01 0.02 0.01\n")
+    assert N.relaxation_batch(f"{d}/bat/b.txt") == 0
+    job = _job(f"{d}/xml", f"""<rbatch><name input="{d}/xml/net" tag="t"/>
+<mech_params stretch="1.0" bend="2e-3"/><rscheme type="ConjGrad" pars="1e-10 100000"/>
+<run><shear min="0.01" max="0.02" inc="0.01"/></run></rbatch>""")
+    assert N.xml_batch(job) == 0
+    assert (open(f"{d}/bat/net0_t_edata.txt").read() ==
+            open(f"{d}/xml/net0_t_edata.txt").read())
