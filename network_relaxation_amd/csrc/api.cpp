@@ -151,44 +151,52 @@ const char *nr_version(void) { return "nrelax 0.1 (gfx950)"; }
 
 int nr_network_read(const char *path, nr_network **out)
 {
-    if (!path || !out) return fail(NR_E_ARG, "null argument");
-    auto h = std::make_unique<nr_network>();
-    std::string err;
-    int rc = nr::read_network(path, h->net, err);
-    if (rc) return fail(rc, err);
-    *out = h.release();
-    return NR_OK;
+    return nr::guarded(NR_E_OPEN, [&]() -> int {
+        if (!path || !out) return fail(NR_E_ARG, "null argument");
+        auto h = std::make_unique<nr_network>();
+        std::string err;
+        int rc = nr::read_network(path, h->net, err);
+        if (rc) return fail(rc, err);
+        *out = h.release();
+        return NR_OK;
+    });
 }
 
 int nr_network_from_arrays(int n_nodes, int n_bonds, double width, double height,
                            const double *pos, const int *bonds, const signed char *wraps,
                            nr_network **out)
 {
-    if (!out) return fail(NR_E_ARG, "null argument");
-    auto h = std::make_unique<nr_network>();
-    std::string err;
-    int rc = nr::network_from_arrays(n_nodes, n_bonds, width, height, pos, bonds, wraps,
-                                     h->net, err);
-    if (rc) return fail(rc, err);
-    *out = h.release();
-    return NR_OK;
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (!out) return fail(NR_E_ARG, "null argument");
+        auto h = std::make_unique<nr_network>();
+        std::string err;
+        int rc = nr::network_from_arrays(n_nodes, n_bonds, width, height, pos, bonds, wraps,
+                                         h->net, err);
+        if (rc) return fail(rc, err);
+        *out = h.release();
+        return NR_OK;
+    });
 }
 
 int nr_network_lattice(int L, double p, uint64_t seed, int lees_edwards, nr_network **out)
 {
-    if (!out) return fail(NR_E_ARG, "null argument");
-    auto h = std::make_unique<nr_network>();
-    std::string err;
-    int rc = nr::make_lattice(L, p, seed, lees_edwards != 0, h->net, err);
-    if (rc) return fail(rc, err);
-    *out = h.release();
-    return NR_OK;
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (!out) return fail(NR_E_ARG, "null argument");
+        auto h = std::make_unique<nr_network>();
+        std::string err;
+        int rc = nr::make_lattice(L, p, seed, lees_edwards != 0, h->net, err);
+        if (rc) return fail(rc, err);
+        *out = h.release();
+        return NR_OK;
+    });
 }
 
 int nr_network_build_triples(nr_network *net)
 {
-    if (!net) return fail(NR_E_ARG, "null network");
-    return nr::build_triples(net->net);
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (!net) return fail(NR_E_ARG, "null network");
+        return nr::build_triples(net->net);
+    });
 }
 
 int nr_network_set_fiber_kappa(nr_network *net, double lo, double hi, uint64_t seed)
@@ -328,100 +336,102 @@ static int tuned_tiles_per_member(const nr_batch *b)
 int nr_batch_create(int count, nr_network *const *nets, const double *ks, const double *kb,
                     const nr_batch_opts *opts, nr_batch **out)
 {
-    if (count <= 0 || !nets || !ks || !kb || !out) return fail(NR_E_ARG, "invalid arguments");
-    auto b = std::make_unique<nr_batch>();
-    b->device = opts ? opts->device : 0;
-    b->precision = opts ? opts->precision : NR_PREC_FP64;
-    if (opts && opts->mixed_switch > 0) b->switch_ratio = opts->mixed_switch;
-    if (b->precision != NR_PREC_FP64 && b->precision != NR_PREC_MIXED)
-        return fail(NR_E_ARG, "unknown precision");
-    HIP_TRY(hipSetDevice(b->device));
-    int64_t lds_budget = 0;
-    int rc = choose_shapes(b.get(), opts, &lds_budget);
-    if (rc) return rc;
-    for (int i = 0; i < count; i++) {
-        if (!nets[i]) return fail(NR_E_ARG, "null network in batch");
-        if (!(ks[i] >= 0) || !(kb[i] >= 0)) return fail(NR_E_ARG, "negative stiffness");
-        b->nets.push_back(&nets[i]->net);
-        b->ks.push_back(ks[i]);
-        b->kb.push_back(kb[i]);
-    }
-    std::string err;
-    rc = nr::build_plan(b->nets, b->ks, b->kb, b->plan.tile_nodes, lds_budget, b->plan, err);
-    if (rc) return fail(rc, err);
-    const nr::Plan &P = b->plan;
-    b->lds = (size_t)(nr::kCacheBytes + P.lds_bytes());
-    if (b->lds > 160 * 1024) return fail(NR_E_TOPO, "tile working set exceeds LDS");
-
-    int per_cu = 0, n_cu = 0;
-    HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
-    if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
-    b->resident = per_cu * n_cu;
-    b->team_cap = std::min(opts && opts->team > 0 ? opts->team : kDefaultTeamCap, b->resident);
-    rc = set_team_layout(b.get(), opts && opts->tiles_per_member > 0 ? opts->tiles_per_member
-                                                               : tuned_tiles_per_member(b.get()));
-    if (rc) return rc;
-
-    // uploads
-    HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreate(&b->ev0));
-    HIP_TRY(hipEventCreate(&b->ev1));
-    const size_t N = (size_t)P.n_total;
-    std::vector<double2> x0(N);
-    for (int k = 0; k < count; k++) {
-        const nr::PlanNet &pn = P.nets[k];
-        const nr::Network &net = *b->nets[k];
-        for (int d = 0; d < pn.n_nodes; d++) {
-            int o = pn.perm[d];
-            x0[pn.node_base + d] = make_double2(net.pos[2 * o], net.pos[2 * o + 1]);
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (count <= 0 || !nets || !ks || !kb || !out) return fail(NR_E_ARG, "invalid arguments");
+        auto b = std::make_unique<nr_batch>();
+        b->device = opts ? opts->device : 0;
+        b->precision = opts ? opts->precision : NR_PREC_FP64;
+        if (opts && opts->mixed_switch > 0) b->switch_ratio = opts->mixed_switch;
+        if (b->precision != NR_PREC_FP64 && b->precision != NR_PREC_MIXED)
+            return fail(NR_E_ARG, "unknown precision");
+        HIP_TRY(hipSetDevice(b->device));
+        int64_t lds_budget = 0;
+        int rc = choose_shapes(b.get(), opts, &lds_budget);
+        if (rc) return rc;
+        for (int i = 0; i < count; i++) {
+            if (!nets[i]) return fail(NR_E_ARG, "null network in batch");
+            if (!(ks[i] >= 0) || !(kb[i] >= 0)) return fail(NR_E_ARG, "negative stiffness");
+            b->nets.push_back(&nets[i]->net);
+            b->ks.push_back(ks[i]);
+            b->kb.push_back(kb[i]);
         }
-    }
-    std::vector<double2> zeros(N, make_double2(0.0, 0.0));
-    HIP_TRY(b->x_init.upload(x0));
-    if (b->precision == NR_PREC_MIXED) {
-        std::vector<float2> z32(N, make_float2(0.0f, 0.0f));
+        std::string err;
+        rc = nr::build_plan(b->nets, b->ks, b->kb, b->plan.tile_nodes, lds_budget, b->plan, err);
+        if (rc) return fail(rc, err);
+        const nr::Plan &P = b->plan;
+        b->lds = (size_t)(nr::kCacheBytes + P.lds_bytes());
+        if (b->lds > 160 * 1024) return fail(NR_E_TOPO, "tile working set exceeds LDS");
+
+        int per_cu = 0, n_cu = 0;
+        HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
+        if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
+        b->resident = per_cu * n_cu;
+        b->team_cap = std::min(opts && opts->team > 0 ? opts->team : kDefaultTeamCap, b->resident);
+        rc = set_team_layout(b.get(), opts && opts->tiles_per_member > 0 ? opts->tiles_per_member
+                                                                   : tuned_tiles_per_member(b.get()));
+        if (rc) return rc;
+
+        // uploads
+        HIP_TRY(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&b->ev0));
+        HIP_TRY(hipEventCreate(&b->ev1));
+        const size_t N = (size_t)P.n_total;
+        std::vector<double2> x0(N);
+        for (int k = 0; k < count; k++) {
+            const nr::PlanNet &pn = P.nets[k];
+            const nr::Network &net = *b->nets[k];
+            for (int d = 0; d < pn.n_nodes; d++) {
+                int o = pn.perm[d];
+                x0[pn.node_base + d] = make_double2(net.pos[2 * o], net.pos[2 * o + 1]);
+            }
+        }
+        std::vector<double2> zeros(N, make_double2(0.0, 0.0));
+        HIP_TRY(b->x_init.upload(x0));
+        if (b->precision == NR_PREC_MIXED) {
+            std::vector<float2> z32(N, make_float2(0.0f, 0.0f));
+            for (int s = 0; s < 2; s++) {
+                HIP_TRY(b->u32[s].upload(z32));
+                HIP_TRY(b->v32[s].upload(z32));
+                HIP_TRY(b->f32[s].upload(z32));
+            }
+        }
         for (int s = 0; s < 2; s++) {
-            HIP_TRY(b->u32[s].upload(z32));
-            HIP_TRY(b->v32[s].upload(z32));
-            HIP_TRY(b->f32[s].upload(z32));
+            HIP_TRY(b->x[s].upload(x0));
+            HIP_TRY(b->v[s].upload(zeros));
+            HIP_TRY(b->f[s].upload(zeros));
         }
-    }
-    for (int s = 0; s < 2; s++) {
-        HIP_TRY(b->x[s].upload(x0));
-        HIP_TRY(b->v[s].upload(zeros));
-        HIP_TRY(b->f[s].upload(zeros));
-    }
-    HIP_TRY(b->mask.upload(P.mask));
-    {
-        std::vector<uint4> blob16(P.blob.size() / 16);
-        if (!blob16.empty()) std::memcpy(blob16.data(), P.blob.data(), P.blob.size());
-        HIP_TRY(b->blob.upload(blob16));
-    }
-    HIP_TRY(b->tiles.upload(P.tiles));
-    b->hnet.resize(count);
-    for (int k = 0; k < count; k++) {
-        nr::NetDev &d = b->hnet[k];
-        std::memset(&d, 0, sizeof(d));
-        const nr::PlanNet &pn = P.nets[k];
-        const nr::Network &net = *b->nets[k];
-        d.node_base = pn.node_base;
-        d.n_nodes = pn.n_nodes;
-        d.tile_base = pn.tile_base;
-        d.n_tiles = pn.n_tiles;
-        d.width = net.width;
-        d.height = net.height;
-        d.miny = net.miny;
-        d.yrange = net.height > 0 ? net.height : net.maxy - net.miny;
-        d.ks = b->ks[k];
-        d.kb = b->kb[k];
-    }
-    HIP_TRY(b->dnet.upload(b->hnet));
-    HIP_TRY(b->dout.alloc(count));
-    HIP_TRY(b->run.alloc(count));
-    HIP_TRY(b->err.alloc(1));
-    HIP_TRY(hipMemset(b->err.p, 0, sizeof(unsigned)));
-    *out = b.release();
-    return NR_OK;
+        HIP_TRY(b->mask.upload(P.mask));
+        {
+            std::vector<uint4> blob16(P.blob.size() / 16);
+            if (!blob16.empty()) std::memcpy(blob16.data(), P.blob.data(), P.blob.size());
+            HIP_TRY(b->blob.upload(blob16));
+        }
+        HIP_TRY(b->tiles.upload(P.tiles));
+        b->hnet.resize(count);
+        for (int k = 0; k < count; k++) {
+            nr::NetDev &d = b->hnet[k];
+            std::memset(&d, 0, sizeof(d));
+            const nr::PlanNet &pn = P.nets[k];
+            const nr::Network &net = *b->nets[k];
+            d.node_base = pn.node_base;
+            d.n_nodes = pn.n_nodes;
+            d.tile_base = pn.tile_base;
+            d.n_tiles = pn.n_tiles;
+            d.width = net.width;
+            d.height = net.height;
+            d.miny = net.miny;
+            d.yrange = net.height > 0 ? net.height : net.maxy - net.miny;
+            d.ks = b->ks[k];
+            d.kb = b->kb[k];
+        }
+        HIP_TRY(b->dnet.upload(b->hnet));
+        HIP_TRY(b->dout.alloc(count));
+        HIP_TRY(b->run.alloc(count));
+        HIP_TRY(b->err.alloc(1));
+        HIP_TRY(hipMemset(b->err.p, 0, sizeof(unsigned)));
+        *out = b.release();
+        return NR_OK;
+    });
 }
 
 int nr_batch_strain(nr_batch *b, int index, double xdisp, double ydisp)
@@ -876,67 +886,69 @@ void nr_batch_destroy(nr_batch *b) { delete b; }
 int nr_autotune(int count, nr_network *const *nets, const double *ks, const double *kb,
                 int probe_steps, nr_batch_opts *opts, double *predicted_ms)
 {
-    if (count <= 0 || !nets || !ks || !kb || !opts || probe_steps < 1)
-        return fail(NR_E_ARG, "invalid arguments");
-    struct Shape {
-        int block, tile;
-    };
-    std::vector<Shape> shapes = {{256, 256}, {512, 512}};
-    if (opts->precision == NR_PREC_FP64) shapes.push_back({256, 512});
-    const std::vector<int> cands = {1, 2, 3, 4, 5, 6, 8, 12, 16};
-    double best = std::numeric_limits<double>::infinity();
-    nr_batch_opts best_opts = *opts;
-    for (const Shape &sh : shapes) {
-        nr_batch_opts o = *opts;
-        o.block = sh.block;
-        o.tile_nodes = sh.tile;
-        o.tiles_per_member = 1;
-        nr_batch *raw = nullptr;
-        int rc = nr_batch_create(count, nets, ks, kb, &o, &raw);
-        if (rc == NR_E_TOPO || rc == NR_E_ARG) continue;  // shape cannot hold these networks
-        if (rc) return rc;
-        std::unique_ptr<nr_batch> b(raw);
-        std::vector<int> tiles(count);
-        std::vector<double> full(count), probe(count, (double)probe_steps);
-        for (int k = 0; k < count; k++) {
-            tiles[k] = b->plan.nets[k].n_tiles;
-            full[k] = nr::expected_steps(b->plan.nets[k].n_nodes, b->plan.nets[k].n_bonds);
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (count <= 0 || !nets || !ks || !kb || !opts || probe_steps < 1)
+            return fail(NR_E_ARG, "invalid arguments");
+        struct Shape {
+            int block, tile;
+        };
+        std::vector<Shape> shapes = {{256, 256}, {512, 512}};
+        if (opts->precision == NR_PREC_FP64) shapes.push_back({256, 512});
+        const std::vector<int> cands = {1, 2, 3, 4, 5, 6, 8, 12, 16};
+        double best = std::numeric_limits<double>::infinity();
+        nr_batch_opts best_opts = *opts;
+        for (const Shape &sh : shapes) {
+            nr_batch_opts o = *opts;
+            o.block = sh.block;
+            o.tile_nodes = sh.tile;
+            o.tiles_per_member = 1;
+            nr_batch *raw = nullptr;
+            int rc = nr_batch_create(count, nets, ks, kb, &o, &raw);
+            if (rc == NR_E_TOPO || rc == NR_E_ARG) continue;  // shape cannot hold these networks
+            if (rc) return rc;
+            std::unique_ptr<nr_batch> b(raw);
+            std::vector<int> tiles(count);
+            std::vector<double> full(count), probe(count, (double)probe_steps);
+            for (int k = 0; k < count; k++) {
+                tiles[k] = b->plan.nets[k].n_tiles;
+                full[k] = nr::expected_steps(b->plan.nets[k].n_nodes, b->plan.nets[k].n_bonds);
+            }
+            const int tpm = opts->tiles_per_member > 0
+                                ? opts->tiles_per_member
+                                : nr::tune_tiles_per_member(tiles, full, b->team_cap, b->resident,
+                                                            cands, nullptr);
+            rc = set_team_layout(b.get(), tpm);
+            if (rc) return rc;
+            nr_relax_params prm;
+            std::memset(&prm, 0, sizeof(prm));
+            prm.alpha0 = 0.1;
+            prm.f_alpha = 0.99;
+            prm.dt_max = 0.1;
+            prm.f_inc = 1.1;
+            prm.f_dec = 0.5;
+            prm.nmin = 5;
+            prm.mass = 1.0;
+            prm.fcut = 0.0;  // never met: every network takes probe_steps steps
+            prm.max_steps = probe_steps;
+            prm.criterion = NR_CRIT_MAX;
+            prm.method = NR_METHOD_FIRE;
+            rc = nr_batch_relax(b.get(), &prm, 1, nullptr);
+            if (rc) return rc;
+            const double units_probe =
+                nr::predict_time(tiles, probe, b->team_cap, b->resident, tpm);
+            const double units_full = nr::predict_time(tiles, full, b->team_cap, b->resident, tpm);
+            const double t = b->last_ms / units_probe * units_full;
+            if (t < best) {
+                best = t;
+                best_opts = o;
+                best_opts.tiles_per_member = tpm;
+            }
         }
-        const int tpm = opts->tiles_per_member > 0
-                            ? opts->tiles_per_member
-                            : nr::tune_tiles_per_member(tiles, full, b->team_cap, b->resident,
-                                                        cands, nullptr);
-        rc = set_team_layout(b.get(), tpm);
-        if (rc) return rc;
-        nr_relax_params prm;
-        std::memset(&prm, 0, sizeof(prm));
-        prm.alpha0 = 0.1;
-        prm.f_alpha = 0.99;
-        prm.dt_max = 0.1;
-        prm.f_inc = 1.1;
-        prm.f_dec = 0.5;
-        prm.nmin = 5;
-        prm.mass = 1.0;
-        prm.fcut = 0.0;  // never met: every network takes probe_steps steps
-        prm.max_steps = probe_steps;
-        prm.criterion = NR_CRIT_MAX;
-        prm.method = NR_METHOD_FIRE;
-        rc = nr_batch_relax(b.get(), &prm, 1, nullptr);
-        if (rc) return rc;
-        const double units_probe =
-            nr::predict_time(tiles, probe, b->team_cap, b->resident, tpm);
-        const double units_full = nr::predict_time(tiles, full, b->team_cap, b->resident, tpm);
-        const double t = b->last_ms / units_probe * units_full;
-        if (t < best) {
-            best = t;
-            best_opts = o;
-            best_opts.tiles_per_member = tpm;
-        }
-    }
-    if (!std::isfinite(best)) return fail(NR_E_TOPO, "no shape holds these networks");
-    *opts = best_opts;
-    if (predicted_ms) *predicted_ms = best;
-    return NR_OK;
+        if (!std::isfinite(best)) return fail(NR_E_TOPO, "no shape holds these networks");
+        *opts = best_opts;
+        if (predicted_ms) *predicted_ms = best;
+        return NR_OK;
+    });
 }
 
 }  // extern "C"

@@ -349,14 +349,16 @@ int relax(nr_composite &c, const nr_relax_params &p, nr_report &rep, double *e_g
 
 extern "C" int nr_mesh_read(const char *path, nr_mesh **out)
 {
-    if (!path || !out) return nr::fail(NR_E_ARG, "null argument");
-    *out = nullptr;
-    auto m = std::make_unique<nr_mesh>();
-    std::string err;
-    int rc = nr::read_mesh(path, m->m, err);
-    if (rc) return nr::fail(rc, err);
-    *out = m.release();
-    return NR_OK;
+    return nr::guarded(NR_E_OPEN, [&]() -> int {
+        if (!path || !out) return nr::fail(NR_E_ARG, "null argument");
+        *out = nullptr;
+        auto m = std::make_unique<nr_mesh>();
+        std::string err;
+        int rc = nr::read_mesh(path, m->m, err);
+        if (rc) return nr::fail(rc, err);
+        *out = m.release();
+        return NR_OK;
+    });
 }
 
 extern "C" int nr_mesh_lattice(int L, double p, uint64_t seed, nr_mesh **out)
@@ -410,26 +412,28 @@ extern "C" void nr_mesh_free(nr_mesh *m) { delete m; }
 extern "C" int nr_composite_create(const nr_mesh *m, const nr_gel_params *g, int device,
                                    nr_composite **out)
 {
-    if (!m || !g || !out) return nr::fail(NR_E_ARG, "null argument");
-    *out = nullptr;
-    auto c = std::make_unique<nr_composite>();
-    c->mesh = m->m;
-    c->gel = *g;
-    c->device = device;
-    const char *g_env = std::getenv("NRELAX_GRAPHS");
-    c->graphs = !(g_env && g_env[0] == '0');
-    std::string err;
-    int rc = nr::mesh_setup(c->mesh, g->ks, g->kb, g->bmod, g->dilation, c->setup, err);
-    if (rc) return nr::fail(rc, err);
-    if (!(g->smod >= 0)) return nr::fail(NR_E_ARG, "smod must be >= 0");
-    HIP_TRY(hipSetDevice(device));
-    HIP_TRY(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreate(&c->e0));
-    HIP_TRY(hipEventCreate(&c->e1));
-    rc = nr::composite_upload(*c);
-    if (rc) return rc;
-    *out = c.release();
-    return NR_OK;
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (!m || !g || !out) return nr::fail(NR_E_ARG, "null argument");
+        *out = nullptr;
+        auto c = std::make_unique<nr_composite>();
+        c->mesh = m->m;
+        c->gel = *g;
+        c->device = device;
+        const char *g_env = std::getenv("NRELAX_GRAPHS");
+        c->graphs = !(g_env && g_env[0] == '0');
+        std::string err;
+        int rc = nr::mesh_setup(c->mesh, g->ks, g->kb, g->bmod, g->dilation, c->setup, err);
+        if (rc) return nr::fail(rc, err);
+        if (!(g->smod >= 0)) return nr::fail(NR_E_ARG, "smod must be >= 0");
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipStreamCreateWithFlags(&c->st, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&c->e0));
+        HIP_TRY(hipEventCreate(&c->e1));
+        rc = nr::composite_upload(*c);
+        if (rc) return rc;
+        *out = c.release();
+        return NR_OK;
+    });
 }
 
 extern "C" int nr_composite_get_info(const nr_composite *c, nr_composite_info *info)
@@ -666,27 +670,29 @@ int run_mesh_protocol(const JobFiles &jf, const MeshBatchSpec &s, bool strain_mo
 // relaxation_batch :1393-1421: every <base>*.nmc of the batch's directory.
 extern "C" int nr_mesh_relaxation_batch(const char *batch_file, int strain_mode, int device)
 {
-    std::string err;
-    nr::MeshBatchSpec s;
-    if (!batch_file) return NR_E_ARG;
-    int rc = nr::parse_mesh_batch(batch_file, s, err);
-    if (rc) {
-        std::fprintf(stderr, "%s\nParsing of the batch file failed.\n", err.c_str());
-        return rc;
-    }
-    std::vector<nr::JobFiles> jobs;
-    rc = nr::file_lists(s.b.base, s.b.tag, jobs, err, "nmc");
-    if (rc) {
-        std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
-        return rc;
-    }
-    int first_err = NR_OK;
-    for (const nr::JobFiles &j : jobs) {
-        rc = nr::run_mesh_protocol(j, s, strain_mode != 0, device, err);
+    return nr::guarded(NR_E_BATCH, [&]() -> int {
+        std::string err;
+        nr::MeshBatchSpec s;
+        if (!batch_file) return NR_E_ARG;
+        int rc = nr::parse_mesh_batch(batch_file, s, err);
         if (rc) {
-            std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
-            if (!first_err) first_err = rc;
+            std::fprintf(stderr, "%s\nParsing of the batch file failed.\n", err.c_str());
+            return rc;
         }
-    }
-    return first_err;
+        std::vector<nr::JobFiles> jobs;
+        rc = nr::file_lists(s.b.base, s.b.tag, jobs, err, "nmc");
+        if (rc) {
+            std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
+            return rc;
+        }
+        int first_err = NR_OK;
+        for (const nr::JobFiles &j : jobs) {
+            rc = nr::run_mesh_protocol(j, s, strain_mode != 0, device, err);
+            if (rc) {
+                std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
+                if (!first_err) first_err = rc;
+            }
+        }
+        return first_err;
+    });
 }

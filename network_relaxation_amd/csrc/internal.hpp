@@ -4,6 +4,8 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <exception>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -21,6 +23,22 @@ namespace nr {
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 void to_report(const NetOut &o, nr_report &r);
+
+// Runs body() and turns an exception escaping it into a status, so that no
+// exception crosses the C ABI: NR_E_ALLOC for host memory, `code` otherwise.
+template <class F>
+int guarded(int code, F &&body) noexcept
+{
+    try {
+        return body();
+    } catch (const std::bad_alloc &) {
+        return fail(NR_E_ALLOC, "host allocation failed");
+    } catch (const std::exception &e) {
+        return fail(code, e.what());
+    } catch (...) {
+        return fail(code, "unexpected exception");
+    }
+}
 
 // Owning device array.
 template <class T>

@@ -160,196 +160,198 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
                        double compression, int affine, double tol, int max_iter, double *u,
                        nr_linear_report *rep)
 {
-    if (!net || !rep || !(tol >= 0.0) || max_iter < 0) return fail(NR_E_ARG, "invalid arguments");
-    const nr::Network &N = net->net;
-    const bool le = N.height > 0;
-    if (le && !affine)
-        return fail(NR_E_ARG, "a periodic box has no top row to displace: use the affine field");
-    if ((kb > 0 || !N.tkb.empty()) && !N.triples_built)
-        return fail(NR_E_ARG, "bending needs the triples (nr_network_build_triples)");
-    const int n2 = 2 * N.n;
-    const Csr K = assemble(N, ks, kb, mu);
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (!net || !rep || !(tol >= 0.0) || max_iter < 0) return fail(NR_E_ARG, "invalid arguments");
+        const nr::Network &N = net->net;
+        const bool le = N.height > 0;
+        if (le && !affine)
+            return fail(NR_E_ARG, "a periodic box has no top row to displace: use the affine field");
+        if ((kb > 0 || !N.tkb.empty()) && !N.triples_built)
+            return fail(NR_E_ARG, "bending needs the triples (nr_network_build_triples)");
+        const int n2 = 2 * N.n;
+        const Csr K = assemble(N, ks, kb, mu);
 
-    // imposed field and its load
-    std::vector<double> bc(n2, 0.0), g, kbc;
-    std::vector<unsigned char> fr(n2);
-    for (int i = 0; i < N.n; i++) {
-        const double y = N.pos[2 * i + 1];
-        if (affine) {
-            bc[2 * i] = shear * (y - N.miny);
-            bc[2 * i + 1] = compression * (N.miny - y);
-        } else if (y > N.maxy - 1e-10) {
-            bc[2 * i] = shear;
-            bc[2 * i + 1] = -compression;
+        // imposed field and its load
+        std::vector<double> bc(n2, 0.0), g, kbc;
+        std::vector<unsigned char> fr(n2);
+        for (int i = 0; i < N.n; i++) {
+            const double y = N.pos[2 * i + 1];
+            if (affine) {
+                bc[2 * i] = shear * (y - N.miny);
+                bc[2 * i + 1] = compression * (N.miny - y);
+            } else if (y > N.maxy - 1e-10) {
+                bc[2 * i] = shear;
+                bc[2 * i + 1] = -compression;
+            }
+            fr[2 * i] = (N.fcode[i] & 1) != 0;
+            fr[2 * i + 1] = (N.fcode[i] & 2) != 0;
         }
-        fr[2 * i] = (N.fcode[i] & 1) != 0;
-        fr[2 * i + 1] = (N.fcode[i] & 2) != 0;
-    }
-    spmv(K, bc, g);
-    kbc = g;
-    const double ix = affine ? shear * N.height : 0.0, iy = affine ? -compression * N.height : 0.0;
-    if (le && affine) {
-        for (int b = 0; b < N.e(); b++) {
-            const int wy = N.bwrap[2 * b + 1];
-            if (!wy) continue;
-            const int i = N.bonds[2 * b], j = N.bonds[2 * b + 1];
-            double dx, dy;
-            bond_geom(N, b, dx, dy);
-            const double l = std::sqrt(dx * dx + dy * dy);
-            const double sk = ks / (l * l * l) * (dx * ix * wy + dy * iy * wy);
-            g[2 * j] += sk * dx;
-            g[2 * j + 1] += sk * dy;
-            g[2 * i] -= sk * dx;
-            g[2 * i + 1] -= sk * dy;
-            if (mu > 0) {
-                const double inv = 1.0 / N.rest[b] / N.rest[b];
-                const double mx = mu * (1.0 - dx * dx * inv) * ix * wy;
-                const double my = mu * (1.0 - dy * dy * inv) * iy * wy;
-                g[2 * j] += mx;
-                g[2 * j + 1] += my;
-                g[2 * i] -= mx;
-                g[2 * i + 1] -= my;
+        spmv(K, bc, g);
+        kbc = g;
+        const double ix = affine ? shear * N.height : 0.0, iy = affine ? -compression * N.height : 0.0;
+        if (le && affine) {
+            for (int b = 0; b < N.e(); b++) {
+                const int wy = N.bwrap[2 * b + 1];
+                if (!wy) continue;
+                const int i = N.bonds[2 * b], j = N.bonds[2 * b + 1];
+                double dx, dy;
+                bond_geom(N, b, dx, dy);
+                const double l = std::sqrt(dx * dx + dy * dy);
+                const double sk = ks / (l * l * l) * (dx * ix * wy + dy * iy * wy);
+                g[2 * j] += sk * dx;
+                g[2 * j + 1] += sk * dy;
+                g[2 * i] -= sk * dx;
+                g[2 * i + 1] -= sk * dy;
+                if (mu > 0) {
+                    const double inv = 1.0 / N.rest[b] / N.rest[b];
+                    const double mx = mu * (1.0 - dx * dx * inv) * ix * wy;
+                    const double my = mu * (1.0 - dy * dy * inv) * iy * wy;
+                    g[2 * j] += mx;
+                    g[2 * j + 1] += my;
+                    g[2 * i] -= mx;
+                    g[2 * i + 1] -= my;
+                }
+            }
+            for (int t = 0; t < N.t(); t++) {
+                const int wya = N.twrap[4 * t + 1], wyb = N.twrap[4 * t + 3];
+                if (!wya && !wyb) continue;
+                const double k = tri_kappa(N, kb, t);
+                const int h = N.tri[3 * t], a = N.tri[3 * t + 1], c = N.tri[3 * t + 2];
+                double dx, dy;
+                arm_geom(N, t, dx, dy);
+                const double l2 = dx * dx + dy * dy;
+                const double f = k * ((ix * (wya + wyb)) * dy - (iy * (wya + wyb)) * dx) / l2;
+                g[2 * h] += -2.0 * dy * f;
+                g[2 * h + 1] += 2.0 * dx * f;
+                g[2 * a] += dy * f;
+                g[2 * a + 1] += -dx * f;
+                g[2 * c] += dy * f;
+                g[2 * c + 1] += -dx * f;
             }
         }
+        // right-hand side; in a periodic box every node is free and the two
+        // translations are null vectors of K, which the load is orthogonal to up
+        // to round-off
+        std::vector<double> b0(n2);
+        for (int k = 0; k < n2; k++) b0[k] = fr[k] ? -g[k] : 0.0;
+        if (le && N.n > 0) {
+            double mx = 0.0, my = 0.0;
+            for (int i = 0; i < N.n; i++) {
+                mx += b0[2 * i];
+                my += b0[2 * i + 1];
+            }
+            mx /= N.n;
+            my /= N.n;
+            for (int i = 0; i < N.n; i++) {
+                b0[2 * i] -= mx;
+                b0[2 * i + 1] -= my;
+            }
+        }
+        const double scale = std::max(std::sqrt(dot(g, g)), std::sqrt(dot(kbc, kbc)));
+        const double rr0 = dot(b0, b0);
+        const double thr2 = (tol * scale) * (tol * scale);
+        const bool done0 = !(rr0 > thr2) || max_iter == 0;
+
+        DevBuf<int> d_rowptr, d_col;
+        DevBuf<double> d_val, d_w, d_r, d_p, d_q, d_part, d_sc;
+        DevBuf<unsigned char> d_free;
+        HIP_TRY(d_rowptr.upload(K.rowptr));
+        HIP_TRY(d_col.upload(K.col));
+        HIP_TRY(d_val.upload(K.val));
+        HIP_TRY(d_free.upload(fr));
+        std::vector<double> zero(n2, 0.0);
+        HIP_TRY(d_w.upload(zero));
+        HIP_TRY(d_r.upload(b0));
+        HIP_TRY(d_p.upload(b0));
+        HIP_TRY(d_q.upload(zero));
+        HIP_TRY(d_part.upload(std::vector<double>(2 * nr::kLinBlocks, 0.0)));
+        HIP_TRY(d_sc.upload({rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0}));
+
+        hipStream_t st = nullptr;
+        hipEvent_t e0 = nullptr, e1 = nullptr;
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int rc = NR_OK;
+        double sc[8] = {rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0};
+        float ms = 0.0f;
+        auto cleanup = [&]() {
+            if (exec) (void)hipGraphExecDestroy(exec);
+            if (graph) (void)hipGraphDestroy(graph);
+            if (e0) (void)hipEventDestroy(e0);
+            if (e1) (void)hipEventDestroy(e1);
+            if (st) (void)hipStreamDestroy(st);
+        };
+        hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreate(&e0);
+        if (e == hipSuccess) e = hipEventCreate(&e1);
+        // a chunk of iterations as one graph (kernels only); replays stop doing
+        // work once the device raises its done flag
+        constexpr int kChunk = 32;  // even: the parity of iteration k repeats per chunk
+        if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
+        if (e == hipSuccess) {
+            for (int c = 0; c < kChunk && e == hipSuccess; c++)
+                e = (hipError_t)nr::launch_cg_iteration(n2, d_rowptr.p, d_col.p, d_val.p, d_free.p,
+                                                        d_w.p, d_r.p, d_p.p, d_q.p, d_part.p, d_sc.p,
+                                                        c, max_iter, st);
+            if (e == hipSuccess) e = (hipError_t)nr::launch_cg_chunk_end(d_sc.p, kChunk, st);
+            hipError_t e2 = hipStreamEndCapture(st, &graph);
+            if (e == hipSuccess) e = e2;
+        }
+        if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (e == hipSuccess) e = hipEventRecord(e0, st);
+        while (e == hipSuccess && sc[3] == 0.0) {
+            e = hipGraphLaunch(exec, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(sc, d_sc.p, sizeof(sc), hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess) e = hipStreamSynchronize(st);
+        }
+        if (e == hipSuccess) e = hipEventRecord(e1, st);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        std::vector<double> w(n2);
+        if (e == hipSuccess)
+            e = hipMemcpy(w.data(), d_w.p, sizeof(double) * n2, hipMemcpyDeviceToHost);
+        cleanup();
+        if (e != hipSuccess) return nr::hip_fail(e, "linear solve");
+
+        // u, the true residual of the reduced system, the linearised energies
+        std::vector<double> kw;
+        spmv(K, w, kw);
+        double res = 0.0;
+        for (int k = 0; k < n2; k++)
+            if (fr[k]) res += (kw[k] + g[k]) * (kw[k] + g[k]);
+        std::vector<double> uf(n2);
+        for (int k = 0; k < n2; k++) uf[k] = bc[k] + w[k];
+        double es = 0.0, eb = 0.0;
+        for (int b = 0; b < N.e(); b++) {
+            const int i = N.bonds[2 * b], j = N.bonds[2 * b + 1];
+            const int wy = N.bwrap[2 * b + 1];
+            double dx, dy;
+            bond_geom(N, b, dx, dy);
+            const double dux = uf[2 * j] + ix * wy - uf[2 * i];
+            const double duy = uf[2 * j + 1] + iy * wy - uf[2 * i + 1];
+            const double de = (dux * dx + duy * dy) / N.rest[b];
+            es += 0.5 * ks * de * de;
+            es += 0.5 * mu * (dux * dux + duy * duy - de * de);
+        }
         for (int t = 0; t < N.t(); t++) {
-            const int wya = N.twrap[4 * t + 1], wyb = N.twrap[4 * t + 3];
-            if (!wya && !wyb) continue;
-            const double k = tri_kappa(N, kb, t);
             const int h = N.tri[3 * t], a = N.tri[3 * t + 1], c = N.tri[3 * t + 2];
+            const int wya = N.twrap[4 * t + 1], wyb = N.twrap[4 * t + 3];
             double dx, dy;
             arm_geom(N, t, dx, dy);
             const double l2 = dx * dx + dy * dy;
-            const double f = k * ((ix * (wya + wyb)) * dy - (iy * (wya + wyb)) * dx) / l2;
-            g[2 * h] += -2.0 * dy * f;
-            g[2 * h + 1] += 2.0 * dx * f;
-            g[2 * a] += dy * f;
-            g[2 * a + 1] += -dx * f;
-            g[2 * c] += dy * f;
-            g[2 * c + 1] += -dx * f;
+            const double dux = uf[2 * a] + ix * wya + uf[2 * c] + ix * wyb - 2 * uf[2 * h];
+            const double duy = uf[2 * a + 1] + iy * wya + uf[2 * c + 1] + iy * wyb - 2 * uf[2 * h + 1];
+            const double cr = dux * dy - duy * dx;
+            eb += 0.5 * tri_kappa(N, kb, t) * cr * cr / l2;
         }
-    }
-    // right-hand side; in a periodic box every node is free and the two
-    // translations are null vectors of K, which the load is orthogonal to up
-    // to round-off
-    std::vector<double> b0(n2);
-    for (int k = 0; k < n2; k++) b0[k] = fr[k] ? -g[k] : 0.0;
-    if (le && N.n > 0) {
-        double mx = 0.0, my = 0.0;
-        for (int i = 0; i < N.n; i++) {
-            mx += b0[2 * i];
-            my += b0[2 * i + 1];
-        }
-        mx /= N.n;
-        my /= N.n;
-        for (int i = 0; i < N.n; i++) {
-            b0[2 * i] -= mx;
-            b0[2 * i + 1] -= my;
-        }
-    }
-    const double scale = std::max(std::sqrt(dot(g, g)), std::sqrt(dot(kbc, kbc)));
-    const double rr0 = dot(b0, b0);
-    const double thr2 = (tol * scale) * (tol * scale);
-    const bool done0 = !(rr0 > thr2) || max_iter == 0;
-
-    DevBuf<int> d_rowptr, d_col;
-    DevBuf<double> d_val, d_w, d_r, d_p, d_q, d_part, d_sc;
-    DevBuf<unsigned char> d_free;
-    HIP_TRY(d_rowptr.upload(K.rowptr));
-    HIP_TRY(d_col.upload(K.col));
-    HIP_TRY(d_val.upload(K.val));
-    HIP_TRY(d_free.upload(fr));
-    std::vector<double> zero(n2, 0.0);
-    HIP_TRY(d_w.upload(zero));
-    HIP_TRY(d_r.upload(b0));
-    HIP_TRY(d_p.upload(b0));
-    HIP_TRY(d_q.upload(zero));
-    HIP_TRY(d_part.upload(std::vector<double>(2 * nr::kLinBlocks, 0.0)));
-    HIP_TRY(d_sc.upload({rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0}));
-
-    hipStream_t st = nullptr;
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int rc = NR_OK;
-    double sc[8] = {rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0};
-    float ms = 0.0f;
-    auto cleanup = [&]() {
-        if (exec) (void)hipGraphExecDestroy(exec);
-        if (graph) (void)hipGraphDestroy(graph);
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        if (st) (void)hipStreamDestroy(st);
-    };
-    hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    // a chunk of iterations as one graph (kernels only); replays stop doing
-    // work once the device raises its done flag
-    constexpr int kChunk = 32;  // even: the parity of iteration k repeats per chunk
-    if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
-    if (e == hipSuccess) {
-        for (int c = 0; c < kChunk && e == hipSuccess; c++)
-            e = (hipError_t)nr::launch_cg_iteration(n2, d_rowptr.p, d_col.p, d_val.p, d_free.p,
-                                                    d_w.p, d_r.p, d_p.p, d_q.p, d_part.p, d_sc.p,
-                                                    c, max_iter, st);
-        if (e == hipSuccess) e = (hipError_t)nr::launch_cg_chunk_end(d_sc.p, kChunk, st);
-        hipError_t e2 = hipStreamEndCapture(st, &graph);
-        if (e == hipSuccess) e = e2;
-    }
-    if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
-    if (e == hipSuccess) e = hipEventRecord(e0, st);
-    while (e == hipSuccess && sc[3] == 0.0) {
-        e = hipGraphLaunch(exec, st);
-        if (e == hipSuccess)
-            e = hipMemcpyAsync(sc, d_sc.p, sizeof(sc), hipMemcpyDeviceToHost, st);
-        if (e == hipSuccess) e = hipStreamSynchronize(st);
-    }
-    if (e == hipSuccess) e = hipEventRecord(e1, st);
-    if (e == hipSuccess) e = hipEventSynchronize(e1);
-    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-    std::vector<double> w(n2);
-    if (e == hipSuccess)
-        e = hipMemcpy(w.data(), d_w.p, sizeof(double) * n2, hipMemcpyDeviceToHost);
-    cleanup();
-    if (e != hipSuccess) return nr::hip_fail(e, "linear solve");
-
-    // u, the true residual of the reduced system, the linearised energies
-    std::vector<double> kw;
-    spmv(K, w, kw);
-    double res = 0.0;
-    for (int k = 0; k < n2; k++)
-        if (fr[k]) res += (kw[k] + g[k]) * (kw[k] + g[k]);
-    std::vector<double> uf(n2);
-    for (int k = 0; k < n2; k++) uf[k] = bc[k] + w[k];
-    double es = 0.0, eb = 0.0;
-    for (int b = 0; b < N.e(); b++) {
-        const int i = N.bonds[2 * b], j = N.bonds[2 * b + 1];
-        const int wy = N.bwrap[2 * b + 1];
-        double dx, dy;
-        bond_geom(N, b, dx, dy);
-        const double dux = uf[2 * j] + ix * wy - uf[2 * i];
-        const double duy = uf[2 * j + 1] + iy * wy - uf[2 * i + 1];
-        const double de = (dux * dx + duy * dy) / N.rest[b];
-        es += 0.5 * ks * de * de;
-        es += 0.5 * mu * (dux * dux + duy * duy - de * de);
-    }
-    for (int t = 0; t < N.t(); t++) {
-        const int h = N.tri[3 * t], a = N.tri[3 * t + 1], c = N.tri[3 * t + 2];
-        const int wya = N.twrap[4 * t + 1], wyb = N.twrap[4 * t + 3];
-        double dx, dy;
-        arm_geom(N, t, dx, dy);
-        const double l2 = dx * dx + dy * dy;
-        const double dux = uf[2 * a] + ix * wya + uf[2 * c] + ix * wyb - 2 * uf[2 * h];
-        const double duy = uf[2 * a + 1] + iy * wya + uf[2 * c + 1] + iy * wyb - 2 * uf[2 * h + 1];
-        const double cr = dux * dy - duy * dx;
-        eb += 0.5 * tri_kappa(N, kb, t) * cr * cr / l2;
-    }
-    rep->e_stretch = es;
-    rep->e_bend = eb;
-    rep->residual = std::sqrt(res);
-    rep->iterations = (int)sc[6];
-    rep->status = !(sc[7] > thr2) ? NR_CONVERGED : NR_NOT_CONVERGED;
-    rep->device_ms = ms;
-    if (u) std::copy(uf.begin(), uf.end(), u);
-    return rc;
+        rep->e_stretch = es;
+        rep->e_bend = eb;
+        rep->residual = std::sqrt(res);
+        rep->iterations = (int)sc[6];
+        rep->status = !(sc[7] > thr2) ? NR_CONVERGED : NR_NOT_CONVERGED;
+        rep->device_ms = ms;
+        if (u) std::copy(uf.begin(), uf.end(), u);
+        return rc;
+    });
 }

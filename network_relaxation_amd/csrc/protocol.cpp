@@ -4,6 +4,7 @@
 // relaxation_batch (:1016-1044), with the same output files and formats.
 #include <algorithm>
 #include <cfloat>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <ctime>
@@ -15,6 +16,7 @@
 #include <vector>
 
 #include "../../include/nrelax.h"
+#include "internal.hpp"
 #include "protocol.hpp"
 
 namespace nr {
@@ -53,6 +55,14 @@ std::vector<std::string> split_ws(const std::string &s)
 
 }  // namespace
 
+bool valid_range(double mn, double mx, double inc)
+{
+    if (!std::isfinite(mn) || !std::isfinite(mx) || !std::isfinite(inc)) return false;
+    if (mn < 0 || mx < 0 || inc < 0 || mx < mn) return false;
+    // a ramp of more increments than this is a mistake, not a protocol
+    return !(inc > 0) || (mx - mn) / inc <= kMaxRampSteps;
+}
+
 int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err)
 {
     std::ifstream in(path);
@@ -71,7 +81,7 @@ int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err)
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
     double klo = 0, khi = 0, seed = 0;
     int nr = std::sscanf(line.c_str(), "%lf %lf %lf %lf %lf", &b.ks, &b.kb, &klo, &khi, &seed);
-    if (nr < 2 || b.ks < 0 || b.kb < 0) { err = "Error: invalid format for mechanical attributes."; return NR_E_BATCH; }
+    if (nr < 2 || !(b.ks >= 0) || !(b.kb >= 0) || !std::isfinite(b.ks) || !std::isfinite(b.kb)) { err = "Error: invalid format for mechanical attributes."; return NR_E_BATCH; }
     if (nr == 5 && klo > 0 && khi >= klo) {
         b.kappa_lo = klo;
         b.kappa_hi = khi;
@@ -82,68 +92,65 @@ int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err)
     int cnt = 0;
     for (const std::string &t : split_ws(line)) {
         double x;
-        if (cnt < 5 && std::sscanf(t.c_str(), "%lf", &x) == 1) b.fp[cnt++] = x;
+        if (cnt < 5 && std::sscanf(t.c_str(), "%lf", &x) == 1) {
+            if (!std::isfinite(x)) { err = "FIRE parameters must be finite."; return NR_E_BATCH; }
+            b.fp[cnt++] = x;
+        }
     }
     if (cnt < 5) { err = "Too few FIRE parameters were supplied."; return NR_E_BATCH; }
 
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
     char w1[64] = {0}, w2[64] = {0};
     nr = std::sscanf(line.c_str(), "%d %lf %lf %d %63s %63s", &b.nmin, &b.mass, &b.fcut, &b.nmax, w1, w2);
-    if (nr < 4) { err = "One or more of nmin, mass, cutoff force and step limit were invalid."; return NR_E_BATCH; }
+    if (nr < 4 || !std::isfinite(b.mass) || !std::isfinite(b.fcut)) { err = "One or more of nmin, mass, cutoff force and step limit were invalid."; return NR_E_BATCH; }
     if (!std::strcmp(w1, "max") || !std::strcmp(w2, "max")) b.criterion = NR_CRIT_MAX;
     if (!std::strcmp(w1, "cg") || !std::strcmp(w2, "cg")) b.method = NR_METHOD_CG;
 
     double mn, mx, inc;
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
     nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
-    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of compression."; return NR_E_BATCH; }
+    if (nr < 3 || !valid_range(mn, mx, inc)) { err = "Error: invalid specification of compression."; return NR_E_BATCH; }
     b.cmin = mn; b.cmax = mx; b.cinc = inc;
 
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
     nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
-    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of shear."; return NR_E_BATCH; }
+    if (nr < 3 || !valid_range(mn, mx, inc)) { err = "Error: invalid specification of shear."; return NR_E_BATCH; }
     b.smin = mn; b.smax = mx; b.sinc = inc;
     return NR_OK;
 }
 
 // get_file_lists, file_lists.cpp:80-137: files of `dir` matching
 // ^(<base>.*)\.<ext>; outputs <dir><m>_<tag>_log.txt, _edata.txt and the
-// displacement base <dir><m>_<tag>.  The four names of one network are kept
-// together (the reference sorts the four lists independently, which can pair
-// one network's input with another's outputs).
+// displacement base <dir><m>_<tag>.  The input is split into '/' tokens
+// (empty tokens dropped, :93-103): the last token is the base, and the others
+// rebuild the directory (a leading '/' kept) only when there are at least two
+// tokens; otherwise the directory is "./" ("/x" searches ./ for x*, "dir/"
+// searches ./ for dir*).  The four names of one network are kept together
+// (the reference sorts the four lists independently, which can pair one
+// network's input with another's outputs: net1 and net10).
 int file_lists(const std::string &input, const std::string &tag, std::vector<JobFiles> &jobs,
                std::string &err, const std::string &ext)
 {
     jobs.clear();
     if (input.empty()) { err = "empty base name"; return NR_E_BATCH; }
-    std::string dir, base;
-    size_t slash = input.find_last_of('/');
-    if (slash == std::string::npos) {
-        dir = "./";
-        base = input;
-    } else {
-        // file_lists.cpp:99-107 rebuilds the directory from '/' tokens
-        std::vector<std::string> toks;
-        std::string cur;
-        for (char c : input) {
-            if (c == '/') {
-                if (!cur.empty()) toks.push_back(cur);
-                cur.clear();
-            } else {
-                cur.push_back(c);
-            }
+    std::vector<std::string> toks;
+    std::string cur;
+    for (char c : input) {
+        if (c == '/') {
+            if (!cur.empty()) toks.push_back(cur);
+            cur.clear();
+        } else {
+            cur.push_back(c);
         }
-        if (!cur.empty()) toks.push_back(cur);
-        base = toks.empty() ? "" : toks.back();
+    }
+    if (!cur.empty()) toks.push_back(cur);
+    if (toks.empty()) { err = "empty base name"; return NR_E_BATCH; }
+    std::string base = toks.back(), dir = "./";
+    if (toks.size() > 1) {
         std::ostringstream oss;
         if (input[0] == '/') oss << "/";
         for (size_t i = 0; i + 1 < toks.size(); i++) oss << toks[i] << "/";
         dir = oss.str();
-        if (input.back() == '/') {  // trailing slash: last token is a directory
-            oss << base << "/";
-            dir = oss.str();
-            base = "";
-        }
     }
     if (base == "*") base = "";
     std::regex expr;
@@ -157,14 +164,27 @@ int file_lists(const std::string &input, const std::string &tag, std::vector<Job
     std::filesystem::path p(dir);
     if (!std::filesystem::exists(p, ec)) return NR_OK;
     std::vector<std::string> names;
-    for (const auto &ent : std::filesystem::directory_iterator(p, ec)) {
-        if (ent.is_directory(ec)) continue;
-        names.push_back(ent.path().filename().string());
+    std::filesystem::directory_iterator it(p, ec), end;
+    for (; !ec && it != end; it.increment(ec)) {
+        std::error_code ed;
+        if (it->is_directory(ed)) continue;
+        names.push_back(it->path().filename().string());
+    }
+    if (ec) {
+        err = "directory " + dir + " could not be listed: " + ec.message();
+        return NR_E_OPEN;
     }
     std::sort(names.begin(), names.end());
     for (const std::string &name : names) {
         std::smatch m;
-        if (!std::regex_search(name, m, expr)) continue;
+        bool hit = false;
+        try {
+            hit = std::regex_search(name, m, expr);
+        } catch (const std::regex_error &) {
+            err = "base name pattern too complex for " + name;
+            return NR_E_BATCH;
+        }
+        if (!hit) continue;
         JobFiles j;
         std::string stem = m[1].str();
         std::string t = tag.empty() ? "" : "_" + tag;
@@ -347,15 +367,16 @@ int parse_linear_batch(const std::string &path, LinearBatchSpec &b, std::string 
     if (tok.size() > 1) b.tag = tok[1];
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
     int nr = std::sscanf(line.c_str(), "%lf %lf %lf", &b.ks, &b.kb, &b.mu);
-    if (nr < 3 || b.ks < 0 || b.kb < 0 || b.mu < 0) { err = "Error: invalid format for mechanical attributes."; return NR_E_BATCH; }
+    if (nr < 3 || !(b.ks >= 0) || !(b.kb >= 0) || !(b.mu >= 0) || !std::isfinite(b.ks) ||
+        !std::isfinite(b.kb) || !std::isfinite(b.mu)) { err = "Error: invalid format for mechanical attributes."; return NR_E_BATCH; }
     double mn, mx, inc;
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
     nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
-    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of compression."; return NR_E_BATCH; }
+    if (nr < 3 || !valid_range(mn, mx, inc)) { err = "Error: invalid specification of compression."; return NR_E_BATCH; }
     b.cmin = mn; b.cmax = mx; b.cinc = inc;
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
     nr = std::sscanf(line.c_str(), "%lf %lf %lf", &mn, &mx, &inc);
-    if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) { err = "Error: invalid specification of shear."; return NR_E_BATCH; }
+    if (nr < 3 || !valid_range(mn, mx, inc)) { err = "Error: invalid specification of shear."; return NR_E_BATCH; }
     b.smin = mn; b.smax = mx; b.sinc = inc;
     return NR_OK;
 }
@@ -456,55 +477,59 @@ int run_linear_protocol(const JobFiles &jf, const LinearBatchSpec &spec, bool st
 
 extern "C" int nr_linear_batch(const char *batch_file, int strain_mode, int affine, int itemize)
 {
-    std::string err;
-    nr::LinearBatchSpec b;
-    if (!batch_file) return NR_E_ARG;
-    int rc = nr::parse_linear_batch(batch_file, b, err);
-    if (rc) {
-        std::fprintf(stderr, "%s\nFile parsing failed.\n", err.c_str());
-        return rc;
-    }
-    std::vector<nr::JobFiles> jobs;
-    rc = nr::file_lists(b.base, b.tag, jobs, err);
-    if (rc) {
-        std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
-        return rc;
-    }
-    int first_err = NR_OK;
-    for (const nr::JobFiles &j : jobs) {
-        rc = nr::run_linear_protocol(j, b, strain_mode != 0, affine != 0, itemize != 0, err);
+    return nr::guarded(NR_E_BATCH, [&]() -> int {
+        std::string err;
+        nr::LinearBatchSpec b;
+        if (!batch_file) return NR_E_ARG;
+        int rc = nr::parse_linear_batch(batch_file, b, err);
         if (rc) {
-            std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
-            if (!first_err) first_err = rc;
+            std::fprintf(stderr, "%s\nFile parsing failed.\n", err.c_str());
+            return rc;
         }
-    }
-    return first_err;
+        std::vector<nr::JobFiles> jobs;
+        rc = nr::file_lists(b.base, b.tag, jobs, err);
+        if (rc) {
+            std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
+            return rc;
+        }
+        int first_err = NR_OK;
+        for (const nr::JobFiles &j : jobs) {
+            rc = nr::run_linear_protocol(j, b, strain_mode != 0, affine != 0, itemize != 0, err);
+            if (rc) {
+                std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
+                if (!first_err) first_err = rc;
+            }
+        }
+        return first_err;
+    });
 }
 
 extern "C" int nr_relaxation_batch(const char *batch_file, int strain_mode,
                                    const nr_batch_opts *opts)
 {
-    std::string err;
-    nr::BatchSpec b;
-    if (!batch_file) return NR_E_ARG;
-    int rc = nr::parse_batch_file(batch_file, b, err);
-    if (rc) {
-        std::fprintf(stderr, "%s\nParsing of the batch file failed.\n", err.c_str());
-        return rc;
-    }
-    std::vector<nr::JobFiles> jobs;
-    rc = nr::file_lists(b.base, b.tag, jobs, err);
-    if (rc) {
-        std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
-        return rc;
-    }
-    int first_err = NR_OK;
-    for (const nr::JobFiles &j : jobs) {
-        rc = nr::run_protocol(j, b, strain_mode != 0, opts, err);
+    return nr::guarded(NR_E_BATCH, [&]() -> int {
+        std::string err;
+        nr::BatchSpec b;
+        if (!batch_file) return NR_E_ARG;
+        int rc = nr::parse_batch_file(batch_file, b, err);
         if (rc) {
-            std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
-            if (!first_err) first_err = rc;
+            std::fprintf(stderr, "%s\nParsing of the batch file failed.\n", err.c_str());
+            return rc;
         }
-    }
-    return first_err;
+        std::vector<nr::JobFiles> jobs;
+        rc = nr::file_lists(b.base, b.tag, jobs, err);
+        if (rc) {
+            std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
+            return rc;
+        }
+        int first_err = NR_OK;
+        for (const nr::JobFiles &j : jobs) {
+            rc = nr::run_protocol(j, b, strain_mode != 0, opts, err);
+            if (rc) {
+                std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
+                if (!first_err) first_err = rc;
+            }
+        }
+        return first_err;
+    });
 }

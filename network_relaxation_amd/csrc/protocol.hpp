@@ -28,6 +28,11 @@ struct JobFiles {
     std::string dat, log, edata, disp;
 };
 
+// Most increments a strain ramp (min max inc) may take.
+constexpr double kMaxRampSteps = 1e7;
+// A ramp's numbers are finite, 0 <= min <= max, inc >= 0, and the ramp takes
+// at most kMaxRampSteps increments.
+bool valid_range(double mn, double mx, double inc);
 int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err);
 // ext: the input extension ("dat" networks, "nmc" network + gel composites)
 int file_lists(const std::string &input, const std::string &tag, std::vector<JobFiles> &jobs,

@@ -616,82 +616,84 @@ extern "C" {
 
 int nr_server_create(const nr_batch_opts *shape, const nr_server_opts *opts, nr_server **out)
 {
-    if (!out) return fail(NR_E_ARG, "null argument");
-    auto s = std::make_unique<nr_server>();
-    s->device = shape ? shape->device : 0;
-    s->precision = shape ? shape->precision : NR_PREC_FP64;
-    if (s->precision != NR_PREC_FP64 && s->precision != NR_PREC_MIXED)
-        return fail(NR_E_ARG, "unknown precision");
-    if (shape && shape->mixed_switch > 0) s->switch_ratio = shape->mixed_switch;
-    s->block = shape && shape->block > 0 ? shape->block : 256;
-    if (s->block != 256 && s->block != 512) return fail(NR_E_ARG, "block must be 256 or 512");
-    s->tile_nodes = shape && shape->tile_nodes > 0 ? shape->tile_nodes : s->block;
-    if (s->tile_nodes > 2 * s->block) return fail(NR_E_ARG, "tile_nodes above 2 x block");
-    if (s->precision == NR_PREC_MIXED && s->tile_nodes > s->block)
-        return fail(NR_E_ARG, "mixed precision needs tile_nodes <= block");
-    s->npt = s->tile_nodes > s->block ? 2 : 1;
-    s->lds_budget = nr::lds_per_workgroup(s->block) - nr::kLdsReserve;
-    s->lds = (size_t)(nr::kCacheBytes + s->lds_budget);
-    // a served mix holds tens of jobs, not hundreds: the widest teams (one
-    // tile per member) give the lowest latency and, measured, the highest
-    // throughput there
-    s->tiles_per_member = shape && shape->tiles_per_member > 0 ? shape->tiles_per_member : 1;
-    const int64_t hbm = opts && opts->hbm_bytes > 0 ? opts->hbm_bytes : (int64_t)8 << 30;
-    s->slot_cap = opts && opts->max_jobs > 0 ? opts->max_jobs : 4096;
-    s->slice_steps = opts && opts->slice_steps > 0 ? opts->slice_steps : 2000;
-    const int threads = opts && opts->planner_threads > 0 ? opts->planner_threads : 4;
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (!out) return fail(NR_E_ARG, "null argument");
+        auto s = std::make_unique<nr_server>();
+        s->device = shape ? shape->device : 0;
+        s->precision = shape ? shape->precision : NR_PREC_FP64;
+        if (s->precision != NR_PREC_FP64 && s->precision != NR_PREC_MIXED)
+            return fail(NR_E_ARG, "unknown precision");
+        if (shape && shape->mixed_switch > 0) s->switch_ratio = shape->mixed_switch;
+        s->block = shape && shape->block > 0 ? shape->block : 256;
+        if (s->block != 256 && s->block != 512) return fail(NR_E_ARG, "block must be 256 or 512");
+        s->tile_nodes = shape && shape->tile_nodes > 0 ? shape->tile_nodes : s->block;
+        if (s->tile_nodes > 2 * s->block) return fail(NR_E_ARG, "tile_nodes above 2 x block");
+        if (s->precision == NR_PREC_MIXED && s->tile_nodes > s->block)
+            return fail(NR_E_ARG, "mixed precision needs tile_nodes <= block");
+        s->npt = s->tile_nodes > s->block ? 2 : 1;
+        s->lds_budget = nr::lds_per_workgroup(s->block) - nr::kLdsReserve;
+        s->lds = (size_t)(nr::kCacheBytes + s->lds_budget);
+        // a served mix holds tens of jobs, not hundreds: the widest teams (one
+        // tile per member) give the lowest latency and, measured, the highest
+        // throughput there
+        s->tiles_per_member = shape && shape->tiles_per_member > 0 ? shape->tiles_per_member : 1;
+        const int64_t hbm = opts && opts->hbm_bytes > 0 ? opts->hbm_bytes : (int64_t)8 << 30;
+        s->slot_cap = opts && opts->max_jobs > 0 ? opts->max_jobs : 4096;
+        s->slice_steps = opts && opts->slice_steps > 0 ? opts->slice_steps : 2000;
+        const int threads = opts && opts->planner_threads > 0 ? opts->planner_threads : 4;
 
-    HIP_TRY(hipSetDevice(s->device));
-    int per_cu = 0, n_cu = 0;
-    HIP_TRY((hipError_t)nr::max_resident_blocks(s->block, s->npt, s->lds, &per_cu, &n_cu));
-    if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
-    s->resident = per_cu * n_cu;
-    s->team_cap = std::min(shape && shape->team > 0 ? shape->team : 512, s->resident);
+        HIP_TRY(hipSetDevice(s->device));
+        int per_cu = 0, n_cu = 0;
+        HIP_TRY((hipError_t)nr::max_resident_blocks(s->block, s->npt, s->lds, &per_cu, &n_cu));
+        if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
+        s->resident = per_cu * n_cu;
+        s->team_cap = std::min(shape && shape->team > 0 ? shape->team : 512, s->resident);
 
-    // arena: per node the FP64 state (x, v, F in two parities), its mask, the
-    // FP32 stage state when mixed, and about 88 B of tile topology
-    const int64_t per_node = 97 + (s->precision == NR_PREC_MIXED ? 48 : 0) + 88;
-    s->node_cap = std::min<int64_t>(hbm / per_node, 0x7fffffffll);
-    s->blob_cap = std::min<int64_t>(s->node_cap * 88 / 16, 0x7fffffffll);
-    s->tile_cap = s->node_cap / 64 + s->slot_cap;
-    s->inc_cap = (int64_t)s->slot_cap * 32;
-    if (s->node_cap < 256) return fail(NR_E_ARG, "HBM budget too small");
-    const size_t N = (size_t)s->node_cap;
-    for (int q = 0; q < 2; q++) {
-        HIP_TRY(s->x[q].alloc(N));
-        HIP_TRY(s->v[q].alloc(N));
-        HIP_TRY(s->f[q].alloc(N));
-        if (s->precision == NR_PREC_MIXED) {
-            HIP_TRY(s->u32[q].alloc(N));
-            HIP_TRY(s->v32[q].alloc(N));
-            HIP_TRY(s->f32[q].alloc(N));
+        // arena: per node the FP64 state (x, v, F in two parities), its mask, the
+        // FP32 stage state when mixed, and about 88 B of tile topology
+        const int64_t per_node = 97 + (s->precision == NR_PREC_MIXED ? 48 : 0) + 88;
+        s->node_cap = std::min<int64_t>(hbm / per_node, 0x7fffffffll);
+        s->blob_cap = std::min<int64_t>(s->node_cap * 88 / 16, 0x7fffffffll);
+        s->tile_cap = s->node_cap / 64 + s->slot_cap;
+        s->inc_cap = (int64_t)s->slot_cap * 32;
+        if (s->node_cap < 256) return fail(NR_E_ARG, "HBM budget too small");
+        const size_t N = (size_t)s->node_cap;
+        for (int q = 0; q < 2; q++) {
+            HIP_TRY(s->x[q].alloc(N));
+            HIP_TRY(s->v[q].alloc(N));
+            HIP_TRY(s->f[q].alloc(N));
+            if (s->precision == NR_PREC_MIXED) {
+                HIP_TRY(s->u32[q].alloc(N));
+                HIP_TRY(s->v32[q].alloc(N));
+                HIP_TRY(s->f32[q].alloc(N));
+            }
         }
-    }
-    HIP_TRY(s->mask.alloc(N));
-    HIP_TRY(s->blob.alloc((size_t)s->blob_cap));
-    HIP_TRY(s->tiles.alloc((size_t)s->tile_cap));
-    HIP_TRY(s->dnet.alloc(s->slot_cap));
-    HIP_TRY(s->dout.alloc(s->slot_cap));
-    HIP_TRY(s->run.alloc(s->slot_cap));
-    HIP_TRY(s->incs.alloc((size_t)s->inc_cap));
-    HIP_TRY(s->inc_out.alloc((size_t)s->inc_cap));
-    HIP_TRY(s->err.alloc(1));
-    HIP_TRY(hipMemset(s->err.p, 0, sizeof(unsigned)));
-    HIP_TRY(hipMemset(s->dout.p, 0, sizeof(nr::NetOut) * s->slot_cap));
-    s->r_nodes.reset(s->node_cap);
-    s->r_tiles.reset(s->tile_cap);
-    s->r_blob.reset(s->blob_cap);
-    s->r_incs.reset(s->inc_cap);
-    s->slot_job.assign(s->slot_cap, -1);
-    for (int k = s->slot_cap - 1; k >= 0; k--) s->free_slots.push_back(k);
-    HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
-    HIP_TRY(hipStreamCreateWithFlags(&s->ustream, hipStreamNonBlocking));
-    HIP_TRY(hipEventCreate(&s->ev0));
-    HIP_TRY(hipEventCreate(&s->ev1));
-    nr_server *raw = s.get();
-    for (int t = 0; t < threads; t++) s->workers.emplace_back([raw] { raw->planner(); });
-    *out = s.release();
-    return NR_OK;
+        HIP_TRY(s->mask.alloc(N));
+        HIP_TRY(s->blob.alloc((size_t)s->blob_cap));
+        HIP_TRY(s->tiles.alloc((size_t)s->tile_cap));
+        HIP_TRY(s->dnet.alloc(s->slot_cap));
+        HIP_TRY(s->dout.alloc(s->slot_cap));
+        HIP_TRY(s->run.alloc(s->slot_cap));
+        HIP_TRY(s->incs.alloc((size_t)s->inc_cap));
+        HIP_TRY(s->inc_out.alloc((size_t)s->inc_cap));
+        HIP_TRY(s->err.alloc(1));
+        HIP_TRY(hipMemset(s->err.p, 0, sizeof(unsigned)));
+        HIP_TRY(hipMemset(s->dout.p, 0, sizeof(nr::NetOut) * s->slot_cap));
+        s->r_nodes.reset(s->node_cap);
+        s->r_tiles.reset(s->tile_cap);
+        s->r_blob.reset(s->blob_cap);
+        s->r_incs.reset(s->inc_cap);
+        s->slot_job.assign(s->slot_cap, -1);
+        for (int k = s->slot_cap - 1; k >= 0; k--) s->free_slots.push_back(k);
+        HIP_TRY(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        HIP_TRY(hipStreamCreateWithFlags(&s->ustream, hipStreamNonBlocking));
+        HIP_TRY(hipEventCreate(&s->ev0));
+        HIP_TRY(hipEventCreate(&s->ev1));
+        nr_server *raw = s.get();
+        for (int t = 0; t < threads; t++) s->workers.emplace_back([raw] { raw->planner(); });
+        *out = s.release();
+        return NR_OK;
+    });
 }
 
 double nr_server_clock(const nr_server *s) { return s ? s->clock() : 0.0; }
@@ -700,41 +702,43 @@ int nr_server_submit(nr_server *s, const nr_network *net, double ks, double kb,
                      const nr_relax_params *params, int n_inc, const double *incs,
                      int keep_positions, int *job)
 {
-    if (!s || !net || !params || n_inc < 1 || !incs) return fail(NR_E_ARG, "invalid arguments");
-    if (!(ks >= 0) || !(kb >= 0)) return fail(NR_E_ARG, "negative stiffness");
-    if (kb > 0 && !net->net.triples_built)
-        return fail(NR_E_ARG, "bending stiffness > 0 but triples were not built");
-    if (params->method != NR_METHOD_FIRE)
-        return fail(NR_E_ARG, "the server relaxes by FIRE (CG runs through nr_batch_relax)");
-    if (!(params->mass > 0) || params->max_steps < 0 || params->fcut < 0)
-        return fail(NR_E_ARG, "invalid relax parameters");
-    if (n_inc > 32 * 64) return fail(NR_E_ARG, "too many increments");
-    for (int i = 0; i < n_inc; i++) {
-        if (!std::isfinite(incs[2 * i]) || !std::isfinite(incs[2 * i + 1]))
-            return fail(NR_E_ARG, "non-finite strain increment");
-        if (net->net.height > 0 && incs[2 * i + 1] != 0.0)
-            return fail(NR_E_ARG, "compression is not defined in the Lees-Edwards box");
-    }
-    auto j = std::make_unique<Job>();
-    j->id = s->next_id++;
-    j->net = net->net;
-    j->ks = ks;
-    j->kb = kb;
-    j->prm = *params;
-    j->incs.resize(n_inc);
-    for (int i = 0; i < n_inc; i++) j->incs[i] = make_double2(incs[2 * i], incs[2 * i + 1]);
-    j->keep_pos = keep_positions != 0;
-    j->t_submit = s->clock();
-    Job *raw = j.get();
-    s->jobs[j->id] = std::move(j);
-    s->waiting.push_back(raw->id);
-    {
-        std::lock_guard<std::mutex> lk(s->mu);
-        s->todo.push_back(raw);
-    }
-    s->cv.notify_one();
-    if (job) *job = raw->id;
-    return NR_OK;
+    return nr::guarded(NR_E_ARG, [&]() -> int {
+        if (!s || !net || !params || n_inc < 1 || !incs) return fail(NR_E_ARG, "invalid arguments");
+        if (!(ks >= 0) || !(kb >= 0)) return fail(NR_E_ARG, "negative stiffness");
+        if (kb > 0 && !net->net.triples_built)
+            return fail(NR_E_ARG, "bending stiffness > 0 but triples were not built");
+        if (params->method != NR_METHOD_FIRE)
+            return fail(NR_E_ARG, "the server relaxes by FIRE (CG runs through nr_batch_relax)");
+        if (!(params->mass > 0) || params->max_steps < 0 || params->fcut < 0)
+            return fail(NR_E_ARG, "invalid relax parameters");
+        if (n_inc > 32 * 64) return fail(NR_E_ARG, "too many increments");
+        for (int i = 0; i < n_inc; i++) {
+            if (!std::isfinite(incs[2 * i]) || !std::isfinite(incs[2 * i + 1]))
+                return fail(NR_E_ARG, "non-finite strain increment");
+            if (net->net.height > 0 && incs[2 * i + 1] != 0.0)
+                return fail(NR_E_ARG, "compression is not defined in the Lees-Edwards box");
+        }
+        auto j = std::make_unique<Job>();
+        j->id = s->next_id++;
+        j->net = net->net;
+        j->ks = ks;
+        j->kb = kb;
+        j->prm = *params;
+        j->incs.resize(n_inc);
+        for (int i = 0; i < n_inc; i++) j->incs[i] = make_double2(incs[2 * i], incs[2 * i + 1]);
+        j->keep_pos = keep_positions != 0;
+        j->t_submit = s->clock();
+        Job *raw = j.get();
+        s->jobs[j->id] = std::move(j);
+        s->waiting.push_back(raw->id);
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->todo.push_back(raw);
+        }
+        s->cv.notify_one();
+        if (job) *job = raw->id;
+        return NR_OK;
+    });
 }
 
 int nr_server_step(nr_server *s, int *n_active, int *n_waiting)

@@ -28,6 +28,8 @@
 // several, run k's names carry "_run<k>" before "_log.txt" / "_edata.txt" /
 // the .pos counter.  A report flag of false skips that file.
 #include <cctype>
+#include <climits>
+#include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -278,8 +280,8 @@ bool number(const XNode &n, const char *a, double &v, bool required, std::string
     char *end = nullptr;
     v = std::strtod(s->c_str(), &end);
     while (end && *end && std::isspace((unsigned char)*end)) end++;
-    if (s->empty() || !end || *end) {
-        err = "line " + std::to_string(n.line) + ": " + a + "=\"" + *s + "\" is not a number";
+    if (s->empty() || !end || *end || !std::isfinite(v)) {
+        err = "line " + std::to_string(n.line) + ": " + a + "=\"" + *s + "\" is not a finite number";
         return false;
     }
     return true;
@@ -303,7 +305,7 @@ bool ramp(const XNode &n, double &mn, double &mx, double &inc, std::string &err)
     if (!number(n, "min", mn, true, err) || !number(n, "max", mx, true, err) ||
         !number(n, "inc", inc, true, err))
         return false;
-    if (mn < 0 || mx < 0 || inc < 0 || mx < mn) {  // as batch lines 5 and 6
+    if (!valid_range(mn, mx, inc)) {  // as batch lines 5 and 6
         err = "line " + std::to_string(n.line) + ": invalid <" + n.name + "> range";
         return false;
     }
@@ -368,8 +370,8 @@ int load_job(const std::string &path, XmlJob &job, std::string &err)
         while (ps >> tok) {
             char *end = nullptr;
             const double v = std::strtod(tok.c_str(), &end);
-            if (!end || *end) {
-                err = "pars: \"" + tok + "\" is not a number";
+            if (!end || *end || !std::isfinite(v)) {
+                err = "pars: \"" + tok + "\" is not a finite number";
                 return NR_E_BATCH;
             }
             pv.push_back(v);
@@ -381,6 +383,10 @@ int load_job(const std::string &path, XmlJob &job, std::string &err)
                   "fcut max_steps";
             return NR_E_BATCH;
         }
+        if (!(pv[5] >= 0 && pv[5] <= INT_MAX) || !(pv[8] >= 0 && pv[8] <= INT_MAX)) {
+            err = "N_min and max_steps must lie in [0, 2^31 - 1]";
+            return NR_E_BATCH;
+        }
         for (int k = 0; k < 5; k++) b.fp[k] = pv[k];
         b.nmin = (int)pv[5];
         b.mass = pv[6];
@@ -389,6 +395,10 @@ int load_job(const std::string &path, XmlJob &job, std::string &err)
     } else if (*type == "ConjGrad") {
         if (pv.size() != 2) {
             err = "ConjGrad pars are \"fcut max_iters\" (or the 9 FIRE numbers)";
+            return NR_E_BATCH;
+        }
+        if (!(pv[1] >= 0 && pv[1] <= INT_MAX)) {
+            err = "max_iters must lie in [0, 2^31 - 1]";
             return NR_E_BATCH;
         }
         const double fire[5] = {0.1, 0.99, 0.1, 1.1, 0.5};
@@ -483,55 +493,59 @@ std::string plan_text(const XmlJob &job, const std::vector<JobFiles> &jobs)
 
 extern "C" int nr_xml_job_plan(const char *job_file, char *plan, int cap)
 {
-    if (!job_file) return nr::fail(NR_E_ARG, "null argument");
-    nr::XmlJob job;
-    std::string err;
-    int rc = nr::load_job(job_file, job, err);
-    if (rc) return nr::fail(rc, err);
-    std::vector<nr::JobFiles> jobs;
-    rc = nr::file_lists(job.spec.base, job.spec.tag, jobs, err);
-    if (rc) return nr::fail(rc, err);
-    const std::string text = nr::plan_text(job, jobs);
-    if (plan && cap > 0) std::snprintf(plan, (size_t)cap, "%s", text.c_str());
-    return (int)text.size() < cap || !plan ? NR_OK : nr::fail(NR_E_ARG, "plan buffer too small");
+    return nr::guarded(NR_E_BATCH, [&]() -> int {
+        if (!job_file) return nr::fail(NR_E_ARG, "null argument");
+        nr::XmlJob job;
+        std::string err;
+        int rc = nr::load_job(job_file, job, err);
+        if (rc) return nr::fail(rc, err);
+        std::vector<nr::JobFiles> jobs;
+        rc = nr::file_lists(job.spec.base, job.spec.tag, jobs, err);
+        if (rc) return nr::fail(rc, err);
+        const std::string text = nr::plan_text(job, jobs);
+        if (plan && cap > 0) std::snprintf(plan, (size_t)cap, "%s", text.c_str());
+        return (int)text.size() < cap || !plan ? NR_OK : nr::fail(NR_E_ARG, "plan buffer too small");
+    });
 }
 
 extern "C" int nr_xml_batch(const char *job_file, int strain_mode, const nr_batch_opts *opts)
 {
-    if (!job_file) return NR_E_ARG;
-    nr::XmlJob job;
-    std::string err;
-    int rc = nr::load_job(job_file, job, err);
-    if (rc) {
-        std::fprintf(stderr, "%s\nParsing of the job file failed.\n", err.c_str());
-        return nr::fail(rc, err);
-    }
-    std::vector<nr::JobFiles> jobs;
-    rc = nr::file_lists(job.spec.base, job.spec.tag, jobs, err);
-    if (rc) {
-        std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
-        return nr::fail(rc, err);
-    }
-    int first_err = NR_OK;
-    for (const nr::JobFiles &j : jobs) {
-        nr::ProtoNet pn;
-        rc = nr::proto_open(j, job.spec, opts, pn, err);
-        for (size_t k = 0; rc == NR_OK && k < job.runs.size(); k++) {
-            nr::BatchSpec b = job.spec;
-            const nr::XmlRun &r = job.runs[k];
-            b.cmin = r.cmin;
-            b.cmax = r.cmax;
-            b.cinc = r.cinc;
-            b.smin = r.smin;
-            b.smax = r.smax;
-            b.sinc = r.sinc;
-            rc = nr::proto_ramps(pn, nr::run_files(j, (int)k + 1, (int)job.runs.size()), b,
-                                 strain_mode != 0, r.report, err);
-        }
+    return nr::guarded(NR_E_BATCH, [&]() -> int {
+        if (!job_file) return NR_E_ARG;
+        nr::XmlJob job;
+        std::string err;
+        int rc = nr::load_job(job_file, job, err);
         if (rc) {
-            std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
-            if (!first_err) first_err = rc;
+            std::fprintf(stderr, "%s\nParsing of the job file failed.\n", err.c_str());
+            return nr::fail(rc, err);
         }
-    }
-    return first_err;
+        std::vector<nr::JobFiles> jobs;
+        rc = nr::file_lists(job.spec.base, job.spec.tag, jobs, err);
+        if (rc) {
+            std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
+            return nr::fail(rc, err);
+        }
+        int first_err = NR_OK;
+        for (const nr::JobFiles &j : jobs) {
+            nr::ProtoNet pn;
+            rc = nr::proto_open(j, job.spec, opts, pn, err);
+            for (size_t k = 0; rc == NR_OK && k < job.runs.size(); k++) {
+                nr::BatchSpec b = job.spec;
+                const nr::XmlRun &r = job.runs[k];
+                b.cmin = r.cmin;
+                b.cmax = r.cmax;
+                b.cinc = r.cinc;
+                b.smin = r.smin;
+                b.smax = r.smax;
+                b.sinc = r.sinc;
+                rc = nr::proto_ramps(pn, nr::run_files(j, (int)k + 1, (int)job.runs.size()), b,
+                                     strain_mode != 0, r.report, err);
+            }
+            if (rc) {
+                std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
+                if (!first_err) first_err = rc;
+            }
+        }
+        return first_err;
+    });
 }

@@ -85,7 +85,21 @@ def test_one_run_keeps_the_batch_names_and_conjgrad_pars(tmp_path):
     ("<rbatch><name input='x'/><mech_params stretch='1' bend='0'/><rscheme type='FIRE' "
      "pars='0.1 0.99'/><run/></rbatch>", "need 9 numbers"),
     ("<rbatch><name input='x'/><mech_params stretch='one' bend='0'/><rscheme type='FIRE' "
-     "pars='" + FIRE_PARS + "'/><run/></rbatch>", "not a number"),
+     "pars='" + FIRE_PARS + "'/><run/></rbatch>", "not a finite number"),
+    ("<rbatch><name input='x'/><mech_params stretch='nan' bend='0'/><rscheme type='FIRE' "
+     "pars='" + FIRE_PARS + "'/><run/></rbatch>", "not a finite number"),
+    ("<rbatch><name input='x'/><mech_params stretch='1' bend='0'/><rscheme type='FIRE' "
+     "pars='0.1 0.99 inf 1.1 0.5 5 1.0 1e-10 1000'/><run/></rbatch>", "not a finite number"),
+    ("<rbatch><name input='x'/><mech_params stretch='1' bend='0'/><rscheme type='FIRE' "
+     "pars='0.1 0.99 0.1 1.1 0.5 1e12 1.0 1e-10 1000'/><run/></rbatch>", "must lie in"),
+    ("<rbatch><name input='x'/><mech_params stretch='1' bend='0'/><rscheme type='ConjGrad' "
+     "pars='1e-10 -5'/><run/></rbatch>", "must lie in"),
+    ("<rbatch><name input='x'/><mech_params stretch='1' bend='0'/><rscheme type='FIRE' "
+     "pars='" + FIRE_PARS + "'/><run><shear min='0' max='1e300' inc='1e-300'/></run></rbatch>",
+     "invalid <shear> range"),
+    ("<rbatch><name input='x'/><mech_params stretch='1' bend='0'/><rscheme type='FIRE' "
+     "pars='" + FIRE_PARS + "'/><run><axial min='0' max='inf' inc='0.1'/></run></rbatch>",
+     "not a finite number"),
     ("<rbatch><name input='x'/><mech_params stretch='1' bend='0'/><rscheme type='FIRE' "
      "pars='" + FIRE_PARS + "'/><run><shear min='0.5' max='0.1' inc='0.1'/></run></rbatch>",
      "invalid <shear> range"),
