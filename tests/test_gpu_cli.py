@@ -24,25 +24,28 @@ def _write_dat(net, path):
             f.write(f"{p} {q} {w}\n")
 
 
-def test_cli_protocol_matches_oracle(tmp_path):
-    print("case cli protocol", flush=True)
+@pytest.mark.parametrize("absolute", [False, True])
+def test_cli_protocol_matches_oracle(tmp_path, absolute):
+    """Strain mode (displacements scaled by the y range, the default) and -a
+    (absolute displacements, strain_mode false: large_strain_relax.cu:837)."""
+    print(f"case cli protocol absolute={absolute}", flush=True)
     gpu_dir, cpu_dir = tmp_path / "gpu", tmp_path / "cpu"
     gpu_dir.mkdir()
     cpu_dir.mkdir()
     net = N.Network.lattice(16, 0.8, 9, False)
-    batch = ("netA t\n1 0.01\n0.1 0.99 0.1 1.1 0.5\n5 1 1e-10 200000\n"
-             "0.01 0.02 0.01\n0.01 0.03 0.01\n")
+    ramps = "0.1 0.2 0.1\n0.1 0.3 0.1\n" if absolute else "0.01 0.02 0.01\n0.01 0.03 0.01\n"
+    batch = "netA t\n1 0.01\n0.1 0.99 0.1 1.1 0.5\n5 1 1e-10 200000\n" + ramps
     for d in (gpu_dir, cpu_dir):
         _write_dat(net, str(d / "netA.dat"))
         (d / "batch.txt").write_text(batch.replace("netA", str(d / "netA")))
-    res = subprocess.run([N.CLI_PATH, "-b", str(gpu_dir / "batch.txt")], capture_output=True,
-                         text=True, timeout=300)
+    args = [N.CLI_PATH, "-b", str(gpu_dir / "batch.txt")] + (["-a"] if absolute else [])
+    res = subprocess.run(args, capture_output=True, text=True, timeout=300)
     assert res.returncode == 0, res.stderr
     rc, base, tag, b = O.parse_batch(str(cpu_dir / "batch.txt"))
     assert rc == 0
     stem = str(cpu_dir / "netA_t")
     assert O.run_protocol(str(cpu_dir / "netA.dat"), stem + "_log.txt", stem + "_edata.txt",
-                          stem, b) == 0
+                          stem, b, strain_mode=not absolute) == 0
     g_log = (gpu_dir / "netA_t_log.txt").read_text().splitlines()
     c_log = (cpu_dir / "netA_t_log.txt").read_text().splitlines()
     assert len(g_log) == len(c_log) == 6
