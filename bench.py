@@ -96,9 +96,11 @@ def main():
     ap.add_argument("--team", type=int, default=0)
     ap.add_argument("--tpm", type=int, default=0, help="tiles per team member")
     ap.add_argument("--switch", type=float, default=0.0, help="mixed FP32 -> FP64 hand-over")
-    ap.add_argument("--plain-sample", type=int, default=8)
-    ap.add_argument("--served", type=int, default=64, help="served requests (0 = skip)")
-    ap.add_argument("--served-lmax", type=int, default=128)
+    ap.add_argument("--plain-sample", type=int, default=64,
+                    help="networks of the batch relaxed in the plain form, one from each of "
+                         "this many strata of equal size in L order (0 = skip)")
+    ap.add_argument("--served", type=int, default=8, help="served requests (0 = skip)")
+    ap.add_argument("--served-lmax", type=int, default=1024)
     ap.add_argument("--served-tpm", type=int, default=0,
                     help="tiles per team member of the server (0 = its default)")
     ap.add_argument("--no-cpu", action="store_true")
@@ -113,7 +115,7 @@ def main():
                     help="also run a device copy of MIB MiB after the timed work, as "
                          "the counter calibration (alone with --steps 0)")
     ap.add_argument("--workload", default="batched",
-                    choices=["batched", "single", "ramp", "large", "composite"],
+                    choices=["batched", "single", "ramp", "large", "composite", "served"],
                     help="batched: the headline (512 uneven networks); single: one "
                          "L=32 network vs the oracle; ramp: L=512, 100 shear increments, "
                          "FIRE and CG; large: L=2048 mixed vs FP64 per increment")
@@ -129,7 +131,7 @@ def main():
             sys.exit(2)
         import network_relaxation_amd as N
         leg = {"single": single_leg, "ramp": ramp_leg, "large": large_leg,
-               "composite": composite_leg}[args.workload]
+               "composite": composite_leg, "served": served_leg}[args.workload]
         res = leg(N, args)
         if args.calibrate_copy:  # after the timed work, for the counter passes
             res["copy_calibration"] = calibrate_copy(args.calibrate_copy)
@@ -195,16 +197,23 @@ def main():
     first_ms = cold_ms[0] if cold_ms else None
     del mixed
 
-    # the same batch in FP64, and the mixed form's energy error against it
+    # the same batch in FP64 (timed like the mixed form: an untimed first
+    # relax, then a relax planned from its step counts), and the mixed form's
+    # energy error against it
     fp64 = None
     if not args.no_fp64:
         b64 = N.Batch(nets, 1.0, 1.0, precision=N.PREC_FP64, **shape)
         progress("batched fp64: created")
-        r64, wall64, kms64 = run_batch(b64, 1, 0)
+        r64, wall64, kms64 = run_batch(b64, 1, 1)
+        cold64 = cold_ms[0]
+        n64 = sum(r["status"] == N.CONVERGED for r in r64)
         e64 = np.array([r["es"] + r["eb"] for r in r64])
         rel = np.abs(e_mixed - e64) / np.abs(e64)
-        fp64 = {"networks_per_s": sum(r["status"] == N.CONVERGED for r in r64) / wall64,
+        fp64 = {"networks_per_s": n64 / wall64,
                 "kernel_ms": kms64,
+                "first_relax": {"kernel_ms": cold64, "networks_per_s": n64 / (cold64 * 1e-3)},
+                "node_steps_per_s": sum(r["steps"] * s["L"] ** 2 for r, s in zip(r64, specs))
+                                    / (kms64 * 1e-3),
                 "mixed_energy_rel_diff": {"median": float(np.median(rel)),
                                           "p90": float(np.percentile(rel, 90)),
                                           "max": float(rel.max()),
@@ -212,24 +221,41 @@ def main():
                                           "n": len(rel)}}
         del b64
 
-    # plain form: one network per launch, FP64, on a sample
+    # plain form: one network per launch, FP64, on a stratified sample: the
+    # batch sorted by L (then p) is cut into plain_sample strata of equal
+    # size and one network is drawn from each, so the sample follows the
+    # batch's size distribution; each sampled time stands for its stratum
+    # (an unbiased estimate of the time to relax the whole batch this way)
     plain = None
     if not args.no_plain and args.plain_sample > 0:
         rng = np.random.RandomState(args.seed + 1)
-        idx = sorted(rng.choice(len(specs), size=min(args.plain_sample, len(specs)), replace=False))
-        t_plain = 0.0
+        order = sorted(range(len(specs)), key=lambda k: (specs[k]["L"], specs[k]["p"]))
+        strata = np.array_split(np.array(order), min(args.plain_sample, len(specs)))
+        idx = [int(rng.choice(st)) for st in strata]
+        t_est = 0.0
+        t_sample = 0.0
         ok = 0
-        for k in idx:
+        plain_node_steps = 0.0
+        for k, st in zip(idx, strata):
             b1 = N.Batch([nets[k]], 1.0, 1.0, tiles_per_member=1)
             b1.shear(0, specs[k]["gamma"])
             ts = time.perf_counter()
             r1 = b1.relax(prm)[0]
-            t_plain += time.perf_counter() - ts
+            dt = time.perf_counter() - ts
+            t_sample += dt
+            t_est += dt * len(st)
+            plain_node_steps += r1["steps"] * specs[k]["L"] ** 2
             ok += r1["status"] == N.CONVERGED
-            progress(f"plain: network {k} (L={specs[k]['L']}) {r1['steps']} steps")
             del b1
-        plain = dict(networks_per_s=ok / t_plain, sample=len(idx),
-                     sample_nodes=int(sum(specs[k]["L"] ** 2 for k in idx)),
+        progress(f"plain: {len(idx)} stratified networks in {t_sample:.1f} s")
+        plain = dict(networks_per_s=len(specs) / t_est * ok / len(idx), sample=len(idx),
+                     sample_wall_s=t_sample, converged=int(ok),
+                     node_steps_per_s=plain_node_steps / t_sample,
+                     sample_mean_nodes=float(np.mean([specs[k]["L"] ** 2 for k in idx])),
+                     batch_mean_nodes=float(np.mean([s["L"] ** 2 for s in specs])),
+                     estimate="one network from each of the strata of equal size of the "
+                              "batch in L order; networks/s = batch size / (sum of sampled "
+                              "times x stratum sizes)",
                      form="one network per launch, FP64, widest team (tiles_per_member=1)")
 
     served = None
@@ -274,9 +300,12 @@ def main():
         "fire_steps": {"mean": float(steps.mean()), "max": int(steps.max()),
                        "fp64_mean": float(np.mean([r["fp64_steps"] for r in reps])),
                        "node_steps": node_steps},
+        "node_steps_per_s": node_steps / (kernel_ms * 1e-3),
         "batched_fp64": fp64,
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["networks_per_s"]) if plain else None,
+        "speedup_vs_plain_first_relax": (n_conv / (first_ms * 1e-3) / plain["networks_per_s"])
+                                        if plain and first_ms else None,
         "served": served,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
@@ -408,11 +437,12 @@ def large_leg(N, args):
 
 
 def served_leg(N, args):
-    """Config-5 style stream through the serving loop: requests with L
-    log-uniform in [16, served_lmax], p U[0.55,0.75], per-fiber kappa, 1-20
-    shear increments of U[0.01,0.05], mixed precision.  First all requests at
-    once (saturated throughput X), then Poisson arrivals at 0.7 X with
-    latency = arrival -> last increment done."""
+    """The served stream through the serving loop: requests with L
+    log-uniform in [16, served_lmax] (1024 by default), p U[0.55,0.75],
+    per-fiber kappa, 1-20 shear increments of U[0.01,0.05], mixed precision.
+    First all requests at once (saturated throughput X), then Poisson
+    arrivals at 0.7 X with latency = arrival -> last increment done; the
+    mixed results of every 4th job against an FP64 ramp of it."""
     rng = np.random.RandomState(args.seed + 5)
     n = args.served
     prm = N.relax_params(fcut=args.fcut, max_steps=args.max_steps, criterion=N.CRIT_MAX, **FIRE)
@@ -468,7 +498,7 @@ def served_leg(N, args):
     progress(f"served: poisson at {lam:.2f}/s done")
     # mixed vs FP64, per increment, on a sample of jobs (FP64 ramp on the GPU)
     errs = []
-    for k in range(0, n, max(1, n // 12)):
+    for k in range(0, n, max(1, n // 4)):
         b = N.Batch([reqs[k][0]], 1.0, 1.0, precision=N.PREC_FP64)
         ramp = b.ramp(reqs[k][1], prm)[0]
         e_m = np.array([x["es"] + x["eb"] for x in res[k]["reports"]])
@@ -476,7 +506,7 @@ def served_leg(N, args):
         errs.extend(list(np.abs(e_m - e_f) / np.abs(e_f)))
         del b
     errs = np.array(errs)
-    return {"requests": n, "converged": int(ok),
+    out = {"requests": n, "converged": int(ok),
             "workload": f"L log-U[16,{args.served_lmax}], p U[0.55,0.75], kappa log-U[1e-5,1e-2] "
                         f"per fiber, 1-20 shear increments U[0.01,0.05], mixed precision, "
                         f"1000-step slices",
@@ -490,6 +520,12 @@ def served_leg(N, args):
             "mixed_vs_fp64_energy_rel": {"median": float(np.median(errs)), "max": float(errs.max()),
                                          "n_over_1e-10": int((errs > 1e-10).sum()),
                                          "n": int(len(errs))}}
+    if args.workload == "served":
+        out.update({"metric": "served relax jobs/s (Poisson arrivals at 0.7 x saturated) and "
+                              "p50/p99 latency", "value": out["networks_per_s"],
+                    "unit": "jobs/s", "n_gpus": 1, "higher_is_better": True,
+                    "dtype": "f32+f64", "data": "synthetic"})
+    return out
 
 
 # HBM bytes per composite FIRE step at L = 512 (FETCH_SIZE x 2 + WRITE_SIZE,
