@@ -176,7 +176,13 @@ int nr_batch_ramp(nr_batch *b, int n_inc, const double *incs,
  * then every network relaxes with params.  nr_batch_graph_replay runs the
  * whole step as one graph launch (the host's network records are uploaded
  * first, so steps added with nr_batch_strain and positions set in between
- * are honoured).  Capturing again replaces the graph. */
+ * are honoured).  Capturing again replaces the graph.
+ * A network relaxed by a team of several workgroups needs the whole team
+ * resident at once: eager relaxes launch such teams cooperatively, but a
+ * graph node is an ordinary launch whose grid only fits the device's resident
+ * workgroups.  Replays therefore need the GPU to themselves; when another
+ * process's kernels hold workgroup slots, a team's bounded waits end the
+ * replay with NR_E_TIMEOUT (after 60 s) instead of hanging. */
 int nr_batch_graph_capture(nr_batch *b, const double *xdisp, const double *ydisp,
                            const nr_relax_params *params, int shared);
 int nr_batch_graph_replay(nr_batch *b, nr_report *reports);
@@ -211,6 +217,18 @@ typedef struct {
 } nr_analysis;
 int nr_batch_analyze(nr_batch *b, int index, double shear_strain, double axial_strain,
                      double k_axial, nr_analysis *out, double *bond_strains);
+/* No counterpart in the reference: nr_batch_analyze for callers that cannot
+ * allocate memory (code running while a HIP graph is captured, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same arguments and results, bit
+ * for bit; its partial sums, bond strains and (when no nr_batch_analyze call
+ * uploaded it before) the network's bond list live in a workspace the library
+ * holds on each GPU it runs on until the process ends (24 MiB + 8 KiB: at
+ * most NR_NOALLOC_MAX_BONDS bonds, NR_E_ARG beyond that), so the call makes
+ * no device or host allocation.  Every call on one GPU shares that workspace:
+ * calls that overlap on one GPU (other threads, other batches) race. */
+#define NR_NOALLOC_MAX_BONDS (1 << 20)
+int nr_batch_analyze_noalloc(nr_batch *b, int index, double shear_strain, double axial_strain,
+                             double k_axial, nr_analysis *out, double *bond_strains);
 /* Device time of the last relax launch in milliseconds. */
 double nr_batch_last_ms(const nr_batch *b);
 

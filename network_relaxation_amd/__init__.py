@@ -146,6 +146,8 @@ def lib():
         L.nr_batch_eval.argtypes = [vp, C.c_int, dp, dp]
         L.nr_batch_analyze.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double,
                                        P(Analysis), dp]
+        L.nr_batch_analyze_noalloc.argtypes = [vp, C.c_int, C.c_double, C.c_double, C.c_double,
+                                               P(Analysis), dp]
         L.nr_linear_response.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_int, C.c_double, C.c_int, dp,
                                          P(LinearReport)]
@@ -372,15 +374,17 @@ class Batch:
         _check(lib().nr_batch_eval(self.h, index, _dp(f), _dp(e)))
         return f, e
 
-    def analyze(self, index, shear, axial=0.0, k_axial=1.0, strains=False):
+    def analyze(self, index, shear, axial=0.0, k_axial=1.0, strains=False, noalloc=False):
         """Post-relax reductions of network `index` on the device
         (nr_batch_analyze): non-affinity against the affine field of `shear`
         and `axial`, axial energy, mean / max |bond strain|, and with
-        strains=True the bond strains in edge order."""
+        strains=True the bond strains in edge order.  noalloc=True calls
+        nr_batch_analyze_noalloc (the library's fixed workspace)."""
         a = Analysis()
         st = np.zeros(max(self.networks[index].info().n_bonds, 1)) if strains else None
-        _check(lib().nr_batch_analyze(self.h, index, shear, axial, k_axial, C.byref(a),
-                                      _dp(st) if strains else None))
+        fn = lib().nr_batch_analyze_noalloc if noalloc else lib().nr_batch_analyze
+        _check(fn(self.h, index, shear, axial, k_axial, C.byref(a),
+                  _dp(st) if strains else None))
         out = dict(nonaffinity=a.nonaffinity, axial_energy=a.axial_energy,
                    mean_strain=a.mean_bond_strain, max_strain=a.max_bond_strain)
         if strains:

@@ -14,9 +14,19 @@
 // not depend on scheduling.
 #include <hip/hip_runtime.h>
 
+#include <mutex>
+
 #include "device.hpp"
 
 namespace nr {
+
+// Workspace of nr_batch_analyze_noalloc, one copy per GPU for the life of the
+// process: the workgroup partials (4 kAnalysisBlocks doubles), then the bond
+// strains (kNoAllocMaxBonds doubles), then a staged bond list
+// (kNoAllocMaxBonds int4 = 2 kNoAllocMaxBonds doubles).  Host code reaches it
+// only through the address hipGetSymbolAddress gives for the current device.
+static __device__ __align__(256) double scratch[kNoAllocScratchDoubles];
+
 namespace {
 
 template <int B>
@@ -83,7 +93,7 @@ __global__ __launch_bounds__(B) void bond_kernel(const double2 *__restrict__ x,
         const double dxf = dxi + uxj - uxi, dyf = dyi + uyj - uyi;
         const double li = sqrt(dxi * dxi + dyi * dyi), lf = sqrt(dxf * dxf + dyf * dyf);
         const double st = (lf - li) / li;
-        strain[e] = st;
+        if (strain) strain[e] = st;
         const double proj = (uxj - uxi) * (dxi / li) + (uyj - uyi) * (dyi / li);
         ax += 0.5 * proj * proj;
         sum += st;
@@ -118,6 +128,25 @@ int launch_analysis(const double2 *x, const double2 *x0, int n, double miny, dou
         e = hipGetLastError();
     }
     return (int)e;
+}
+
+// Device address of `scratch` on the current device, looked up once per
+// device (each GPU holds its own copy at its own address).
+int analysis_scratch(int device, double **ptr)
+{
+    constexpr int kMaxDevices = 64;
+    static std::mutex mu;
+    static double *addr[kMaxDevices] = {};
+    if (device < 0 || device >= kMaxDevices) return (int)hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(mu);
+    if (!addr[device]) {
+        double *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return (int)e;
+        addr[device] = p;
+    }
+    *ptr = addr[device];
+    return 0;
 }
 
 }  // namespace nr

@@ -788,6 +788,27 @@ int nr_batch_set_positions(nr_batch *b, int index, const double *pos)
     return NR_OK;
 }
 
+// Workgroup partials (non-affinity, then axial energy, strain sum, max
+// |strain| per workgroup) added in workgroup order; the bond kernel does not
+// run without bonds, so its partials are not read then.
+static void analysis_result(const double *hp, int n_nodes, int ne, double k_axial,
+                            nr_analysis *out)
+{
+    double na = 0.0, ax = 0.0, sum = 0.0, mx = 0.0;
+    for (int w = 0; w < nr::kAnalysisBlocks; w++) {
+        na += hp[w];
+        if (ne > 0) {
+            ax += hp[nr::kAnalysisBlocks + 3 * w];
+            sum += hp[nr::kAnalysisBlocks + 3 * w + 1];
+            mx = std::max(mx, hp[nr::kAnalysisBlocks + 3 * w + 2]);
+        }
+    }
+    out->nonaffinity = n_nodes > 0 ? na / n_nodes : 0.0;
+    out->axial_energy = k_axial * ax;
+    out->mean_bond_strain = ne > 0 ? sum / ne : 0.0;
+    out->max_bond_strain = mx;
+}
+
 int nr_batch_analyze(nr_batch *b, int index, double shear_strain, double axial_strain,
                      double k_axial, nr_analysis *out, double *bond_strains)
 {
@@ -824,18 +845,67 @@ int nr_batch_analyze(nr_batch *b, int index, double shear_strain, double axial_s
         HIP_TRY(hipMemcpyAsync(bond_strains, strain.p, sizeof(double) * ne,
                                hipMemcpyDeviceToHost, b->stream));
     HIP_TRY(hipStreamSynchronize(b->stream));
-    // workgroup partials added in workgroup order
-    double na = 0.0, ax = 0.0, sum = 0.0, mx = 0.0;
-    for (int w = 0; w < nr::kAnalysisBlocks; w++) {
-        na += hp[w];
-        ax += hp[nr::kAnalysisBlocks + 3 * w];
-        sum += hp[nr::kAnalysisBlocks + 3 * w + 1];
-        mx = std::max(mx, hp[nr::kAnalysisBlocks + 3 * w + 2]);
+    analysis_result(hp.data(), pn.n_nodes, ne, k_axial, out);
+    return NR_OK;
+}
+
+// nr_batch_analyze with its workspace in the library's fixed device array
+// (analysis.hip): no device or host allocation.  The bond list comes from an
+// earlier nr_batch_analyze call when there was one, else it is staged into
+// the workspace from the network in chunks through a stack buffer.
+int nr_batch_analyze_noalloc(nr_batch *b, int index, double shear_strain, double axial_strain,
+                             double k_axial, nr_analysis *out, double *bond_strains)
+{
+    if (!b || !out || index < 0 || index >= (int)b->hnet.size())
+        return fail(NR_E_ARG, "invalid arguments");
+    const nr::Network &net = *b->nets[index];
+    const nr::PlanNet &pn = b->plan.nets[index];
+    const nr::NetDev &nd = b->hnet[index];
+    const int ne = net.e();
+    if (ne > nr::kNoAllocMaxBonds)
+        return fail(NR_E_ARG, "nr_batch_analyze_noalloc: " + std::to_string(ne) +
+                                  " bonds exceed its workspace of " +
+                                  std::to_string(nr::kNoAllocMaxBonds));
+    HIP_TRY(hipSetDevice(b->device));
+    double *ws = nullptr;
+    HIP_TRY((hipError_t)nr::analysis_scratch(b->device, &ws));
+    double *part = ws, *strain = ws + 4 * nr::kAnalysisBlocks;
+    int4 *staged = reinterpret_cast<int4 *>(strain + nr::kNoAllocMaxBonds);
+    const int4 *bonds = nullptr;
+    if (ne > 0) {
+        if (b->an_bonds.size() == b->hnet.size() && b->an_bonds[index]) {
+            bonds = b->an_bonds[index]->p;
+        } else {
+            constexpr int kChunk = 1024;
+            int4 chunk[kChunk];
+            for (int e0 = 0; e0 < ne; e0 += kChunk) {
+                const int m = std::min(kChunk, ne - e0);
+                for (int i = 0; i < m; i++) {
+                    const int e = e0 + i;
+                    chunk[i] = make_int4(pn.iperm[net.bonds[2 * e]], pn.iperm[net.bonds[2 * e + 1]],
+                                         net.bwrap[2 * e], net.bwrap[2 * e + 1]);
+                }
+                HIP_TRY(hipMemcpyAsync(staged + e0, chunk, sizeof(int4) * m,
+                                       hipMemcpyHostToDevice, b->stream));
+                // the stack chunk is reused: wait for its copy
+                HIP_TRY(hipStreamSynchronize(b->stream));
+            }
+            bonds = staged;
+        }
     }
-    out->nonaffinity = pn.n_nodes > 0 ? na / pn.n_nodes : 0.0;
-    out->axial_energy = k_axial * ax;
-    out->mean_bond_strain = ne > 0 ? sum / ne : 0.0;
-    out->max_bond_strain = mx;
+    const double2 *x = b->x[nd.parity].p + pn.node_base;
+    const double2 *x0 = b->x_init.p + pn.node_base;
+    HIP_TRY((hipError_t)nr::launch_analysis(x, x0, pn.n_nodes, net.miny, shear_strain,
+                                            axial_strain, bonds, ne, net.width, net.height,
+                                            nd.le_shift, bond_strains ? strain : nullptr, part,
+                                            b->stream));
+    double hp[4 * nr::kAnalysisBlocks];
+    HIP_TRY(hipMemcpyAsync(hp, part, sizeof(hp), hipMemcpyDeviceToHost, b->stream));
+    if (bond_strains && ne > 0)
+        HIP_TRY(hipMemcpyAsync(bond_strains, strain, sizeof(double) * ne,
+                               hipMemcpyDeviceToHost, b->stream));
+    HIP_TRY(hipStreamSynchronize(b->stream));
+    analysis_result(hp, pn.n_nodes, ne, k_axial, out);
     return NR_OK;
 }
 

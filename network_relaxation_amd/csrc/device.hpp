@@ -162,8 +162,15 @@ int launch_cg_chunk_end(double *sc, int chunk, void *stream);
 int launch_spmv(int n, const int *rowptr, const int *col, const double *val,
                 const unsigned char *free_dof, const double *x, double *y, double *part,
                 void *stream);
+// strain may be null (the bond strains are then not stored).
 int launch_analysis(const double2 *x, const double2 *x0, int n, double miny, double shear,
                     double axial, const int4 *bonds, int ne, double W, double H, double shift,
                     double *strain, double *part, void *stream);
+// Fixed workspace of the allocation-free analysis (analysis.hip): partials,
+// then kNoAllocMaxBonds bond strains, then a staged list of as many bonds.
+constexpr int kNoAllocMaxBonds = NR_NOALLOC_MAX_BONDS;
+constexpr size_t kNoAllocScratchDoubles = 4 * kAnalysisBlocks + 3 * (size_t)kNoAllocMaxBonds;
+// Device address of that workspace on `device`, which must be current.
+int analysis_scratch(int device, double **ptr);
 
 }  // namespace nr
