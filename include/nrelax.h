@@ -47,7 +47,11 @@ extern "C" {
 #define NR_CRIT_MAX 1        /* max over nodes of |F_node| < fcut */
 
 #define NR_METHOD_FIRE 0
-#define NR_METHOD_CG 1
+#define NR_METHOD_CG 1       /* nonlinear CG, Fletcher-Reeves (the method the
+                                reference names, large_strain_types.hpp:31-34) */
+#define NR_METHOD_CG_PR 2    /* nonlinear CG, Polak-Ribiere with beta =
+                                max(0, F.(F - F_old) / |F_old|^2): a restart
+                                whenever that is not positive */
 
 #define NR_PREC_FP64 0       /* FP64 throughout */
 #define NR_PREC_MIXED 1      /* FP32 forces on FP64 anchor + FP32 displacement,
@@ -72,7 +76,7 @@ typedef struct {
     double fcut;
     int max_steps;
     int criterion;     /* NR_CRIT_RMS or NR_CRIT_MAX */
-    int method;        /* NR_METHOD_FIRE or NR_METHOD_CG */
+    int method;        /* NR_METHOD_FIRE, NR_METHOD_CG or NR_METHOD_CG_PR */
 } nr_relax_params;
 
 /* Result of one relax: report_data[3] and step_count of run_fire_md

@@ -22,7 +22,7 @@ OK = 0
  E_MESH) = range(1, 13)
 CONVERGED, NOT_CONVERGED, NONFINITE, ABORTED, IN_PROGRESS = range(5)
 CRIT_RMS, CRIT_MAX = 0, 1
-METHOD_FIRE, METHOD_CG = 0, 1
+METHOD_FIRE, METHOD_CG, METHOD_CG_PR = 0, 1, 2
 PREC_FP64, PREC_MIXED = 0, 1
 
 
@@ -479,9 +479,11 @@ def autotune(networks, ks, kb, precision=PREC_FP64, probe_steps=200, device=0, t
                 tiles_per_member=opts.tiles_per_member), pred.value
 
 
-def relaxation_batch(batch_file, strain_mode=True, precision=PREC_FP64, tile_nodes=0):
-    """nr_relaxation_batch: the reference program's whole protocol."""
-    opts = BatchOpts(precision, 0, tile_nodes, 0, 0, 0, 0.0)
+def relaxation_batch(batch_file, strain_mode=True, precision=PREC_FP64, tile_nodes=0,
+                     tiles_per_member=0):
+    """nr_relaxation_batch: the reference program's whole protocol, every
+    matched network in one device batch."""
+    opts = BatchOpts(precision, 0, tile_nodes, 0, 0, tiles_per_member, 0.0)
     return lib().nr_relaxation_batch(os.fsencode(batch_file), 1 if strain_mode else 0,
                                      C.byref(opts))
 

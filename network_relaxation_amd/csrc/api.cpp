@@ -533,7 +533,7 @@ static int set_params(nr_batch *b, const nr_relax_params *params, int shared)
     b->method = params[0].method;
     for (size_t k = 0; k < b->hnet.size(); k++) {
         const nr_relax_params &p = shared ? params[0] : params[k];
-        if (p.method != NR_METHOD_FIRE && p.method != NR_METHOD_CG)
+        if (p.method != NR_METHOD_FIRE && p.method != NR_METHOD_CG && p.method != NR_METHOD_CG_PR)
             return fail(NR_E_ARG, "unknown relax method");
         if (!(p.mass > 0) || p.max_steps < 0 || p.fcut < 0)
             return fail(NR_E_ARG, "invalid relax parameters");

@@ -105,6 +105,7 @@ int parse_batch_file(const std::string &path, BatchSpec &b, std::string &err)
     if (nr < 4 || !std::isfinite(b.mass) || !std::isfinite(b.fcut)) { err = "One or more of nmin, mass, cutoff force and step limit were invalid."; return NR_E_BATCH; }
     if (!std::strcmp(w1, "max") || !std::strcmp(w2, "max")) b.criterion = NR_CRIT_MAX;
     if (!std::strcmp(w1, "cg") || !std::strcmp(w2, "cg")) b.method = NR_METHOD_CG;
+    if (!std::strcmp(w1, "cgpr") || !std::strcmp(w2, "cgpr")) b.method = NR_METHOD_CG_PR;
 
     double mn, mx, inc;
     if (!read_line(in, line)) { err = "Not enough information was provided in the batch file."; return NR_E_BATCH; }
@@ -340,7 +341,7 @@ int proto_ramps(ProtoNet &pn, const JobFiles &jf, const BatchSpec &b, bool strai
     return rc;
 }
 
-// do_relaxation_run :792-1010.
+// do_relaxation_run :792-1010 for one network.
 int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
                  const nr_batch_opts *opts, std::string &err)
 {
@@ -348,6 +349,209 @@ int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
     int rc = proto_open(jf, b, opts, pn, err);
     if (rc) return rc;
     return proto_ramps(pn, jf, b, strain_mode, kReportAll, err);
+}
+
+namespace {
+
+// Nodes of the networks one device batch of relaxation_batch holds at once
+// (about 10 GB of device state at 64 M nodes).
+constexpr int64_t kProtocolBatchNodes = int64_t(1) << 26;
+
+// One network of a batched protocol run: its files, its ramp in its own
+// displacement units, and where its loops stand.
+struct BatchedNet {
+    const JobFiles *jf = nullptr;
+    nr_network *n = nullptr;
+    nr_network_info info{};
+    double range = 0, cmn = 0, cmx = 0, cinc = 0, smn = 0, smx = 0, sinc = 0, cur = 0;
+    double max_comp = 0;
+    int disp_count = 1;
+    FILE *lf = nullptr, *ef = nullptr;
+    bool active = false;
+};
+
+void close_net(BatchedNet &z, std::clock_t ct)
+{
+    if (z.lf) {
+        std::fprintf(z.lf, "Elapsed time: %f seconds\n", ((float)ct) / CLOCKS_PER_SEC);
+        std::fclose(z.lf);
+    }
+    if (z.ef) std::fclose(z.ef);
+    z.lf = z.ef = nullptr;
+}
+
+// The compression then shear ramps of do_relaxation_run (:917-1000) for
+// every network of `nets` at once: each round applies every active
+// network's increment (scaled by its own y range in strain mode) and relaxes
+// them all in one persistent launch; a network whose loop has ended sits out
+// (step limit 0: its state is left as it is).  Outputs are each network's
+// own, in the reference's formats.
+int ramps_batched(std::vector<BatchedNet> &nets, nr_batch *bt, const BatchSpec &b,
+                  std::string &err)
+{
+    const int count = (int)nets.size();
+    nr_relax_params rp;
+    rp.alpha0 = b.fp[0];
+    rp.f_alpha = b.fp[1];
+    rp.dt_max = b.fp[2];
+    rp.f_inc = b.fp[3];
+    rp.f_dec = b.fp[4];
+    rp.nmin = b.nmin;
+    rp.mass = b.mass;
+    rp.fcut = b.fcut;
+    rp.max_steps = b.nmax;
+    rp.criterion = b.criterion;
+    rp.method = b.method;
+    std::vector<nr_relax_params> prm(count, rp);
+    std::vector<nr_report> rep(count);
+    std::vector<double> pos;
+    for (int pass = 0; pass < 2; pass++) {  // compression, then shear
+        const bool shear = pass == 1;
+        for (BatchedNet &z : nets) {
+            const double mn = shear ? z.smn : z.cmn, mx = shear ? z.smx : z.cmx;
+            z.cur = mn;
+            z.active = mx > 0 && z.cur <= mx + kFltTol;
+        }
+        for (;;) {
+            bool any = false;
+            for (int k = 0; k < count; k++) {
+                BatchedNet &z = nets[k];
+                prm[k].max_steps = z.active ? b.nmax : 0;
+                if (!z.active) continue;
+                any = true;
+                const int rc = shear ? nr_batch_strain(bt, k, z.sinc, 0.0)
+                                     : nr_batch_strain(bt, k, 0.0, -z.cinc);
+                if (rc) {
+                    err = nr_last_error();
+                    return rc;
+                }
+            }
+            if (!any) break;
+            int rc = nr_batch_relax(bt, prm.data(), 0, rep.data());
+            if (rc) {
+                err = nr_last_error();
+                return rc;
+            }
+            for (int k = 0; k < count; k++) {
+                BatchedNet &z = nets[k];
+                if (!z.active) continue;
+                log_result(z.lf, rep[k], b.fcut, b.criterion);
+                if (!shear) {
+                    if (z.ef)
+                        std::fprintf(z.ef, "%12.0lf\t%12.10lf\t%12.8le\t%12.8le\t%12.8le\n", 0.,
+                                     z.cur, rep[k].e_stretch, rep[k].e_bend,
+                                     rep[k].e_stretch + rep[k].e_bend);
+                } else {
+                    if (z.ef)
+                        std::fprintf(z.ef, "%12.10lf\t%12.10lf\t%12.10le\t%12.10le\t%12.10le\n",
+                                     z.cur, z.max_comp, rep[k].e_stretch, rep[k].e_bend,
+                                     rep[k].e_stretch + rep[k].e_bend);
+                    pos.resize(2 * (size_t)z.info.n_nodes);
+                    if ((rc = nr_batch_get_positions(bt, k, pos.data())) != 0) {
+                        err = nr_last_error();
+                        return rc;
+                    }
+                    const int digits =
+                        dec_digits(1 + (b.sinc > 0 ? (int)((b.smax - b.smin) / b.sinc) : 0));
+                    const std::string name = report_name(z.jf->disp, digits, z.disp_count++, "pos");
+                    if (FILE *pf = std::fopen(name.c_str(), "w")) {
+                        for (int i = 0; i < z.info.n_nodes; i++)
+                            std::fprintf(pf, "%2.10le\t%2.10le\n", pos[2 * i], pos[2 * i + 1]);
+                        std::fclose(pf);
+                    }
+                }
+                z.cur += shear ? z.sinc : z.cinc;
+                z.active = z.cur <= (shear ? z.smx : z.cmx) + kFltTol;
+            }
+        }
+        if (!shear)  // :957 max_comp from the loop's exit value
+            for (BatchedNet &z : nets) z.max_comp = z.cmx > z.cmn ? z.cur - z.cinc : z.cmx;
+    }
+    return NR_OK;
+}
+
+}  // namespace
+
+// relaxation_batch (:1016-1044) with every network of the directory in one
+// resident device batch (in groups of at most kProtocolBatchNodes nodes):
+// per network the files and numbers of do_relaxation_run, the increments of
+// all networks relaxed together.  A network that cannot be read or set up is
+// reported and skipped, as the reference moves on to the next file.  The
+// elapsed time in a network's log is that of its group.  With a fixed
+// tiles_per_member in opts each network's arithmetic, and so its files, do
+// not depend on the other networks.
+int run_protocol_batched(const std::vector<JobFiles> &jobs, const BatchSpec &b, bool strain_mode,
+                         const nr_batch_opts *opts, int &first_err)
+{
+    first_err = NR_OK;
+    auto note = [&](const JobFiles &j, int rc, const std::string &msg) {
+        std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), msg.c_str());
+        if (!first_err) first_err = rc;
+    };
+    if ((b.cmax > 0 && !(b.cinc > 0)) || (b.smax > 0 && !(b.sinc > 0))) {
+        for (const JobFiles &j : jobs)
+            note(j, NR_E_BATCH, "a strain ramp with a maximum above 0 needs a positive increment");
+        return first_err;
+    }
+    size_t next = 0;
+    while (next < jobs.size()) {
+        std::clock_t ct = std::clock();
+        std::vector<BatchedNet> nets;
+        std::vector<nr_network *> handles;
+        int64_t nodes = 0;
+        for (; next < jobs.size() && (nets.empty() || nodes < kProtocolBatchNodes); next++) {
+            const JobFiles &j = jobs[next];
+            BatchedNet z;
+            z.jf = &j;
+            int rc = nr_network_read(j.dat.c_str(), &z.n);
+            if (rc) {
+                note(j, rc, nr_last_error());
+                continue;
+            }
+            if (b.kb > 0) {
+                nr_network_build_triples(z.n);
+                if (b.kappa_hi > 0)
+                    nr_network_set_fiber_kappa(z.n, b.kappa_lo, b.kappa_hi, b.kappa_seed);
+            }
+            nr_network_get_info(z.n, &z.info);
+            if (z.info.height > 0 && b.cmax > 0) {
+                note(j, NR_E_BATCH, "compression is not defined in the Lees-Edwards box");
+                nr_network_free(z.n);
+                continue;
+            }
+            z.range = z.info.height > 0 ? z.info.height : z.info.maxy - z.info.miny;
+            const double smult = strain_mode ? z.range : 1.0;
+            z.cmn = b.cmin * smult;
+            z.cmx = b.cmax * smult;
+            z.cinc = b.cinc * smult;
+            z.smn = b.smin * smult;
+            z.smx = b.smax * smult;
+            z.sinc = b.sinc * smult;
+            nodes += z.info.n_nodes;
+            handles.push_back(z.n);
+            nets.push_back(z);
+        }
+        if (nets.empty()) continue;
+        std::vector<double> ks(nets.size(), b.ks), kb(nets.size(), b.kb);
+        nr_batch *bt = nullptr;
+        int rc = nr_batch_create((int)nets.size(), handles.data(), ks.data(), kb.data(), opts, &bt);
+        std::string err = rc ? nr_last_error() : "";
+        if (!rc) {
+            for (BatchedNet &z : nets) {
+                z.lf = std::fopen(z.jf->log.c_str(), "w");
+                z.ef = std::fopen(z.jf->edata.c_str(), "w");
+            }
+            rc = ramps_batched(nets, bt, b, err);
+            nr_batch_destroy(bt);
+        }
+        ct = std::clock() - ct;
+        for (BatchedNet &z : nets) {
+            close_net(z, ct);
+            if (rc) note(*z.jf, rc, err);
+            nr_network_free(z.n);
+        }
+    }
+    return first_err;
 }
 
 // parse_batch_file of sparse_relax.cpp (:920-1035).
@@ -523,13 +727,7 @@ extern "C" int nr_relaxation_batch(const char *batch_file, int strain_mode,
             return rc;
         }
         int first_err = NR_OK;
-        for (const nr::JobFiles &j : jobs) {
-            rc = nr::run_protocol(j, b, strain_mode != 0, opts, err);
-            if (rc) {
-                std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
-                if (!first_err) first_err = rc;
-            }
-        }
+        nr::run_protocol_batched(jobs, b, strain_mode != 0, opts, first_err);
         return first_err;
     });
 }

@@ -41,6 +41,10 @@ int dec_digits(int value);
 std::string report_name(const std::string &base, int digits, int count, const std::string &ext);
 int run_protocol(const JobFiles &jf, const BatchSpec &b, bool strain_mode,
                  const nr_batch_opts *opts, std::string &err);
+// relaxation_batch: every network of jobs in one device batch (groups bounded
+// by node count); first_err receives the first failure's code.
+int run_protocol_batched(const std::vector<JobFiles> &jobs, const BatchSpec &b, bool strain_mode,
+                         const nr_batch_opts *opts, int &first_err);
 
 // Outputs of a ramp (the XML job format's report flags).
 constexpr unsigned kReportEdata = 1, kReportLog = 2, kReportDisp = 4, kReportAll = 7;

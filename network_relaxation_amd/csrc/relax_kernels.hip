@@ -35,7 +35,7 @@ namespace {
 // kPassCgTrial is one line-search trial of nonlinear CG: direction
 // d = f + beta d_old (first trial of an iteration; later trials reload it),
 // trial point x + a d from the iteration's base point, forces there, and the
-// reduction (F.d, -, |F|^2, max |F_node|^2).
+// reduction (F.d, F.F_base, |F|^2, max |F_node|^2).
 enum PassKind { kPassInit = 0, kPassStep = 1, kPassEnergy = 2, kPassFuse = 3, kPassCgTrial = 4 };
 enum MixKind { kMixNone = 0, kMixFire = 1, kMixZero = 2 };
 // CG line search: at most this many secant trials, accepted when
@@ -1149,9 +1149,12 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                     if (!(mk[j] & 2)) fn.y = 0.0;
                     st_state(sh, A.f[q] + gid, fn);
                     if (KIND == kPassCgTrial) {
-                        // F . d along the trial direction
+                        // F . d along the trial direction, and F . F_base
+                        // (the Polak-Ribiere numerator's cross term)
                         acc.a = fma(fn.x, vh[j].x, acc.a);
                         acc.a = fma(fn.y, vh[j].y, acc.a);
+                        acc.b = fma(fn.x, fo[j].x, acc.b);
+                        acc.b = fma(fn.y, fo[j].y, acc.b);
                     } else {
                         double2 vn;
                         vn.x = fma(S.dtm, fn.x, vh[j].x);  // :677 second half kick
@@ -1364,12 +1367,14 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
         shift = C.shift;
 
         if (CG && phase == kRunCg && status != NR_ABORTED) {
-            // Nonlinear CG (Fletcher-Reeves, secant line search on
-            // phi'(a) = -F(x + a d).d): the algorithm of the FP64 oracle's CG.
+            // Nonlinear CG (Fletcher-Reeves or Polak-Ribiere+, secant line
+            // search on phi'(a) = -F(x + a d).d): the algorithm of the FP64
+            // oracle's CG.
             // f.d and |d|^2 of the new direction d = f + beta d_old follow from
             // the last accepted trial's reductions.
             const double fcut = nd->fcut;
             const int crit = nd->crit;
+            const bool pr = nd->method == NR_METHOD_CG_PR;
             const double sqrt_dof = sqrt((double)(2 * C.n_nodes));
             double beta_next, step_len, dd_old, fdt_acc;
             int pd;
@@ -1441,7 +1446,9 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 step_len = a * sqrt(dd);
                 p ^= 1;
                 pd ^= 1;
-                beta_next = tr.c / ff;
+                // Fletcher-Reeves |F|^2 / |F_old|^2, or Polak-Ribiere+
+                // max(0, (|F|^2 - F.F_old) / |F_old|^2)
+                beta_next = pr ? fmax(0.0, (tr.c - tr.b) / ff) : tr.c / ff;
                 fdt_acc = tr.a;
                 dd_old = dd;
                 ff = tr.c;
@@ -1674,7 +1681,7 @@ __global__ void zero_words_kernel(unsigned long long *p, size_t n)
 
 const void *kernel_for(int block, int npt, int method)
 {
-    const bool cg = method == NR_METHOD_CG;
+    const bool cg = method == NR_METHOD_CG || method == NR_METHOD_CG_PR;
     if (block == 256 && npt == 1)
         return cg ? (const void *)cg_kernel<256, 1> : (const void *)relax_kernel<256, 1>;
     if (block == 256 && npt == 2)

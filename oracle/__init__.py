@@ -101,7 +101,7 @@ def lib():
         L.or_fire.argtypes = [P(OrNet), C.c_double, C.c_double, dp, dp, C.c_int,
                               C.c_double, C.c_double, C.c_int, C.c_int, dp, P(C.c_int)]
         L.or_cg.argtypes = [P(OrNet), C.c_double, C.c_double, dp, C.c_double, C.c_int,
-                            C.c_int, dp, P(C.c_int)]
+                            C.c_int, C.c_int, dp, P(C.c_int)]
         L.or_parse_batch.argtypes = [C.c_char_p, C.c_char_p, C.c_int, C.c_char_p,
                                      C.c_int, P(OrBatch)]
         L.or_run_protocol.argtypes = [C.c_char_p, C.c_char_p, C.c_char_p, C.c_char_p,
@@ -352,11 +352,12 @@ class Net:
         return dict(nonaffinity=out[0], axial_energy=out[1], mean_strain=out[2],
                     max_strain=out[3], strains=st[:self.s.e])
 
-    def cg(self, ks, kb, pos, fcut=1e-10, max_iters=1000000, criterion=CRIT_MAX):
+    def cg(self, ks, kb, pos, fcut=1e-10, max_iters=1000000, criterion=CRIT_MAX, pr=False):
+        """Nonlinear CG: Fletcher-Reeves, or Polak-Ribiere+ with pr=True."""
         rep = (C.c_double * 4)()
         it = C.c_int()
         rc = lib().or_cg(C.byref(self.s), ks, kb, _dp(pos), fcut, max_iters, criterion,
-                         rep, C.byref(it))
+                         1 if pr else 0, rep, C.byref(it))
         return dict(rc=rc, rms=rep[0], es=rep[1], eb=rep[2], fmax=rep[3], steps=it.value)
 
 

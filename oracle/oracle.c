@@ -626,13 +626,15 @@ int or_fire_core(int n, or_force_fn fn, const void *ctx, double *pos, const doub
 
 /* Nonlinear conjugate gradient (extension: the reference only names the
  * method, large_strain_types.hpp:31-34, "Fletcher-Reeve").  Fletcher-Reeves
- * directions, restart to steepest descent when d is not a descent direction,
+ * directions, or Polak-Ribiere+ (beta = max(0, F.(F - F_old) / |F_old|^2), a
+ * restart whenever that is not positive), restart to steepest descent when d
+ * is not a descent direction,
  * secant line search on phi'(a) = -F(x + a d).d, trial points always formed
  * from the base point.  Each iteration first tests convergence. */
 #define OR_CG_LS_MAX 12
 #define OR_CG_ETA 1e-3
 int or_cg(const or_net *net, double ks, double kb, double *pos, double fcut,
-          int max_iters, int criterion, double report[4], int *iters)
+          int max_iters, int criterion, int pr, double report[4], int *iters)
 {
     int ndof = 2 * net->n, i, it = 0, rc = OR_OK;
     double sqrt_dof = sqrt((double)ndof), ff, fmax2, gg, step_len = 1e-2;
@@ -646,7 +648,7 @@ int or_cg(const or_net *net, double ks, double kb, double *pos, double fcut,
     gg = ff;
     memcpy(d, f, sizeof(double) * (size_t)ndof);
     while (it < max_iters) {
-        double fd = 0.0, dd = 0.0, g0, a_prev, g_prev, a, g = 0.0, ggn;
+        double fd = 0.0, dd = 0.0, g0, a_prev, g_prev, a, g = 0.0, ggn, fof, beta;
         int ls;
         if (criterion == OR_CRIT_MAX ? sqrt(fmax2) < fcut : sqrt(ff) / sqrt_dof < fcut)
             break;
@@ -685,11 +687,16 @@ int or_cg(const or_net *net, double ks, double kb, double *pos, double fcut,
             or_forces(net, ks, kb, xt, ft);
         }
         step_len = a * sqrt(dd);
+        /* Polak-Ribiere numerator's cross term F_new . F_old */
+        fof = 0.0;
+        for (i = 0; i < ndof; i++) fof = fma(ft[i], f[i], fof);
         memcpy(pos, xt, sizeof(double) * (size_t)ndof);
         memcpy(f, ft, sizeof(double) * (size_t)ndof);
         or_norms(net->n, f, &ff, &fmax2);
         ggn = ff;
-        for (i = 0; i < ndof; i++) d[i] = fma(ggn / gg, d[i], f[i]);
+        /* Fletcher-Reeves |F|^2/|F_old|^2 or Polak-Ribiere+ max(0, ...) */
+        beta = pr ? fmax(0.0, (ggn - fof) / gg) : ggn / gg;
+        for (i = 0; i < ndof; i++) d[i] = fma(beta, d[i], f[i]);
         gg = ggn;
     }
     report[0] = sqrt(ff) / sqrt_dof;
@@ -758,6 +765,7 @@ int or_parse_batch(const char *path, char *base, int base_cap, char *tag,
     if (nr < 4) goto out;
     if (strcmp(w1, "max") == 0 || strcmp(w2, "max") == 0) b->criterion = OR_CRIT_MAX;
     if (strcmp(w1, "cg") == 0 || strcmp(w2, "cg") == 0) b->method = 1;
+    if (strcmp(w1, "cgpr") == 0 || strcmp(w2, "cgpr") == 0) b->method = 2;
     if (!or_read_line(fp, &ln, &eof)) goto out;
     nr = sscanf(ln.buf, "%lf %lf %lf", &mn, &mx, &inc);
     if (nr < 3 || mn < 0 || mx < 0 || inc < 0 || mx < mn) goto out;
@@ -793,8 +801,9 @@ void or_report_name(const char *base, int num_digits, int count,
 static int or_relax(const or_net *net, const or_batch *b, double *pos,
                     double rep[4], int *steps)
 {
-    if (b->method == 1)
-        return or_cg(net, b->ks, b->kb, pos, b->fcut, b->nmax, b->criterion, rep, steps);
+    if (b->method == 1 || b->method == 2)
+        return or_cg(net, b->ks, b->kb, pos, b->fcut, b->nmax, b->criterion, b->method == 2,
+                     rep, steps);
     return or_fire(net, b->ks, b->kb, pos, b->fp, b->nmin, b->mass, b->fcut,
                    b->nmax, b->criterion, rep, steps);
 }

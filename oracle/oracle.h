@@ -94,8 +94,10 @@ double or_shear_virial(const or_net *net, double ks, double kb, const double *po
 int or_fire(const or_net *net, double ks, double kb, double *pos,
             const double fp[5], int nmin, double mass, double fcut,
             int max_steps, int criterion, double report[4], int *steps);
+/* Nonlinear CG: Fletcher-Reeves (pr = 0) or Polak-Ribiere+ (pr != 0,
+ * beta = max(0, F.(F - F_old) / |F_old|^2)). */
 int or_cg(const or_net *net, double ks, double kb, double *pos, double fcut,
-          int max_iters, int criterion, double report[4], int *iters);
+          int max_iters, int criterion, int pr, double report[4], int *iters);
 
 /* Whole strain protocol with the reference's output files (strain_mode as
  * the reference's default, absolute displacements when 0). */

@@ -1,4 +1,5 @@
-"""GPU nonlinear CG (Fletcher-Reeves, secant line search) through the C ABI.
+"""GPU nonlinear CG (Fletcher-Reeves and Polak-Ribiere+, secant line search)
+through the C ABI.
 
 The reference only names the method (large_strain_types.hpp:31-34); the FP64
 oracle's CG is the algorithm both follow.  The device forms f.d and |d|^2 of
@@ -24,13 +25,15 @@ def _say(msg):
     sys.stdout.flush()
 
 
-@pytest.mark.parametrize("L,le", [(16, True), (24, False), (64, True)])
-def test_cg_matches_oracle_cg_and_fire_minimum(L, le):
-    _say(f"case cg L={L} le={le}")
+@pytest.mark.parametrize("L,le,pr", [(16, True, False), (24, False, False), (64, True, False),
+                                     (16, True, True), (24, False, True), (64, True, True)])
+def test_cg_matches_oracle_cg_and_fire_minimum(L, le, pr):
+    """Fletcher-Reeves (pr False) and Polak-Ribiere+ (pr True)."""
+    _say(f"case cg L={L} le={le} pr={pr}")
     o = O.Net.lattice(L, 0.7, 40 + L, le).triples()
     n = N.Network.lattice(L, 0.7, 40 + L, le).triples()
     kb = 4e-3
-    cg = N.relax_params(fcut=1e-10, method=N.METHOD_CG)
+    cg = N.relax_params(fcut=1e-10, method=N.METHOD_CG_PR if pr else N.METHOD_CG)
     b = N.Batch([n], 1.0, kb)
     b.shear(0, 0.02)
     r = b.relax(cg)[0]
@@ -38,7 +41,7 @@ def test_cg_matches_oracle_cg_and_fire_minimum(L, le):
     pos = o.pos()
     o.shear(pos, 0.02)
     pos_cg = pos.copy()
-    ro = o.cg(1.0, kb, pos_cg, fcut=1e-10)
+    ro = o.cg(1.0, kb, pos_cg, fcut=1e-10, pr=pr)
     assert ro["rc"] == 0
     et, eo = r["es"] + r["eb"], ro["es"] + ro["eb"]
     assert abs(et - eo) <= 1e-12 * eo, (et, eo)

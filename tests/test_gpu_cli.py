@@ -66,6 +66,46 @@ def test_cli_protocol_matches_oracle(tmp_path, absolute):
         np.testing.assert_allclose(gp, cp, rtol=0, atol=1e-8)
 
 
+def test_batched_directory_equals_each_network_alone(tmp_path):
+    """relaxation_batch relaxes every <base>*.dat of the directory in one
+    device batch (large_strain_relax.cu:1038-1043 loops over them one at a
+    time).  With tiles_per_member fixed, each network's arithmetic does not
+    depend on the others: four networks of different heights (so different
+    displacement units in strain mode) relaxed together write, byte for byte,
+    the files each writes when it is the only network, Elapsed-time lines
+    aside.  A file that cannot be read is reported and skipped."""
+    print("case batched directory vs alone", flush=True)
+    specs = [("netA", 16, 0.8, 9), ("netB", 24, 0.75, 3), ("netC", 12, 0.85, 5),
+             ("netD", 20, 0.7, 8)]
+    batch = ("{base} t\n1 0.01\n0.1 0.99 0.1 1.1 0.5\n5 1 1e-10 200000\n"
+             "0.01 0.02 0.01\n0.01 0.03 0.01\n")
+    alld = tmp_path / "all"
+    alld.mkdir()
+    for name, L, p, seed in specs:
+        net = N.Network.lattice(L, p, seed, False)
+        _write_dat(net, str(alld / f"{name}.dat"))
+        one = tmp_path / name
+        one.mkdir()
+        _write_dat(net, str(one / f"{name}.dat"))
+        (one / "b.txt").write_text(batch.format(base=one / name))
+    (alld / "netE.dat").write_text("3 1 5\n0 0\n")  # too few points: skipped
+    (alld / "b.txt").write_text(batch.format(base=alld / "net"))
+    assert N.relaxation_batch(str(alld / "b.txt"), tiles_per_member=2) == N.E_POINTS
+    for name, *_ in specs:
+        assert N.relaxation_batch(str(tmp_path / name / "b.txt"), tiles_per_member=2) == 0
+        files = sorted(f.name for f in (tmp_path / name).iterdir()
+                       if f.name.startswith(name + "_t"))
+        assert len(files) == 2 + 3
+        for f in files:
+            a = (alld / f).read_text().splitlines()
+            b = (tmp_path / name / f).read_text().splitlines()
+            if f.endswith("_log.txt"):
+                assert a[-1].startswith("Elapsed time:") and b[-1].startswith("Elapsed time:")
+                a, b = a[:-1], b[:-1]
+            assert a == b, f
+    assert not (alld / "netE_t_log.txt").exists()
+
+
 @pytest.mark.parametrize("itemize", [False, True])
 def test_linear_cli_matches_oracle(tmp_path, itemize):
     """nrelax_linear (the sp_relax drop-in, strain mode) against the oracle's

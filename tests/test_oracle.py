@@ -308,21 +308,25 @@ def test_fire_reports_non_convergence_at_step_limit():
 
 # ---- CG (extension) ---------------------------------------------------------
 
-def test_cg_agrees_with_fire():
+@pytest.mark.parametrize("pr", [False, True])
+def test_cg_agrees_with_fire(pr):
     n = O.Net.lattice(8, 0.75, 3, True).triples()
     pos = n.pos()
     n.shear(pos, 0.02)
     p1, p2 = pos.copy(), pos.copy()
     rf = n.fire(1.0, 1e-2, p1, fcut=1e-11)
-    rc = n.cg(1.0, 1e-2, p2, fcut=1e-11)
+    rc = n.cg(1.0, 1e-2, p2, fcut=1e-11, pr=pr)
     assert rc["rc"] == 0 and rc["fmax"] < 1e-11
     assert abs((rf["es"] + rf["eb"]) - (rc["es"] + rc["eb"])) <= 1e-10 * (rf["es"] + rf["eb"])
 
 
-def test_cg_quadratic_chain_converges_in_n_steps():
+@pytest.mark.parametrize("pr", [False, True])
+def test_cg_quadratic_chain_converges_in_n_steps(pr):
     """Collinear chain stretched along x between clamped ends: the energy is
     quadratic in the x displacements of the free nodes, so CG with an exact
-    (secant) line search reaches the minimum in at most that many steps."""
+    (secant) line search reaches the minimum in at most that many steps; on a
+    quadratic successive forces are orthogonal, so Polak-Ribiere+ takes the
+    Fletcher-Reeves directions and the same bound holds for it."""
     m = 6
     pos = [[i * 1.0, 0.0 if i in (0, m) else 1.0] for i in range(m + 1)]
     # y of the end nodes is 0 and 2 -> clamps; free nodes at y = 1 but then
@@ -333,7 +337,7 @@ def test_cg_quadratic_chain_converges_in_n_steps():
     p = np.array(pos, float).reshape(-1).copy()
     p[1::2][1:m] += 0.1 * np.sin(np.arange(1, m))
     p[2 * m + 1] = m * 1.2   # stretched top end
-    r = n.cg(1.0, 0.0, p, fcut=1e-12, criterion=O.CRIT_MAX)
+    r = n.cg(1.0, 0.0, p, fcut=1e-12, criterion=O.CRIT_MAX, pr=pr)
     assert r["rc"] == 0 and r["steps"] <= m - 1 + 1
     np.testing.assert_allclose(p[1::2], np.arange(m + 1) * 1.2, atol=1e-11)
 
