@@ -125,6 +125,8 @@ struct nr_batch {
 
     void drop_graph()
     {
+        // a replay that returned early may still be queued on the stream
+        if (gexec && stream) (void)hipStreamSynchronize(stream);
         if (gexec) (void)hipGraphExecDestroy(gexec);
         if (graph) (void)hipGraphDestroy(graph);
         gexec = nullptr;

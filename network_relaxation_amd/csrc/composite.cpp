@@ -65,6 +65,9 @@ struct nr_composite {
 
     ~nr_composite()
     {
+        // a relax that returned early (an error, the step bound) may have
+        // left graph replays queued: they finish before the graph goes
+        if (st) (void)hipStreamSynchronize(st);
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
         if (e0) (void)hipEventDestroy(e0);
@@ -259,12 +262,16 @@ hipError_t reset_state(nr_composite &c, const nr_relax_params &p)
 
 hipError_t capture_chunk(nr_composite &c)
 {
+    // nothing may still run the graph being replaced (a relax that returned
+    // early can leave replays queued on the stream)
+    hipError_t e = hipStreamSynchronize(c.st);
+    if (e != hipSuccess) return e;
     if (c.exec) (void)hipGraphExecDestroy(c.exec);
     if (c.graph) (void)hipGraphDestroy(c.graph);
     c.exec = nullptr;
     c.graph = nullptr;
     c.have_graph = false;
-    hipError_t e = hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal);
+    e = hipStreamBeginCapture(c.st, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     // the chunk starts reading slot 1 (the first drift writes it)
     for (int k = 0; k < kMeshChunk && e == hipSuccess; k++)
