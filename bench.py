@@ -40,10 +40,17 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per relax launch of the default workload (512 networks, seed 2024,
 # L 16..192, |F|max < 1e-9, tuned shapes; a relax planned from the previous
-# one's step counts), from rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs
-# scaled by what those counters read for a 1 GiB device copy (fetch x 2.0,
-# write x 1.0): 12,135,584,362 KiB x 2 + 6,454,443,609 KiB.
-PROFILED_TRAFFIC_BYTES = (12135584362 * 2 + 6454443609) * 1024.0
+# one's step counts), from the relax kernel's own request counters
+# (rocprofv3 --pmc): reads 32 RDREQ_32B + 64 RDREQ_64B + 128 RDREQ_128B =
+# 64 x 191,477,218 + 128 x 193,991,669,988, writes 64 WRREQ_64B + 32 (WRREQ -
+# WRREQ_64B) = 64 x 103,026,182,191 + 32 x 489,789,357 (the same sums give
+# exactly 1 GiB for a 1 GiB device copy).
+PROFILED_TRAFFIC_BYTES = (64 * 191477218 + 128 * 193991669988 +
+                          64 * 103026182191 + 32 * (103515971548 - 103026182191))
+# The same counters on one L=512 network's FIRE ramp (p 0.6, kappa 1e-4):
+# counted HBM bytes per FIRE step (5.5875e12 read + 1.6761e12 written over
+# 131,826 steps).
+RAMP512_TRAFFIC_PER_STEP = (64 * 314571609 + 128 * 43495102686 + 1636776740 * 1024) / 131826
 FIRE = dict(alpha0=0.1, f_alpha=0.99, dt_max=0.5, f_inc=1.1, f_dec=0.5, nmin=5, mass=1.0)
 
 
@@ -383,7 +390,14 @@ def ramp_leg(N, args):
         steps = sum(r["steps"] for r in reps)
         n_, e_, t_ = b.counts(0)
         byt = steps * alg_bytes(n_, e_, t_, per_triple_kappa=False)
-        out[name] = {"node_updates_per_s": nodes * steps / (kms * 1e-3),
+        achieved = byt / (kms * 1e-3) / 1e9
+        out[name] = {"roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                                  "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                                  "traffic": RAMP512_TRAFFIC_PER_STEP * steps
+                                  if name == "fire" else None,
+                                  "note": "traffic: counted bytes per FIRE step of this "
+                                          "network x steps"},
+                     "node_updates_per_s": nodes * steps / (kms * 1e-3),
                      "steps_per_increment": [r["steps"] for r in reps],
                      "steps": steps, "kernel_ms": kms, "wall_s": wall,
                      "converged": sum(r["status"] == N.CONVERGED for r in reps),
