@@ -114,6 +114,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-linear", action="store_true", help="skip the GPU linear solves")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="skip the autotuned, graph-replayed strain step of the batch")
     ap.add_argument("--no-composite", action="store_true",
                     help="skip the network + gel composite relax")
     ap.add_argument("--composite-L", type=int, default=512)
@@ -206,6 +208,8 @@ def main():
     e_mixed = np.array([r["es"] + r["eb"] for r in reps])
     first_ms = cold_ms[0] if cold_ms else None
     del mixed
+
+    graph = None if args.no_graph else graph_leg(N, nets, specs, prm, args, shape)
 
     # the same batch in FP64 (timed like the mixed form: an untimed first
     # relax, then a relax planned from its step counts), and the mixed form's
@@ -311,6 +315,7 @@ def main():
                        "fp64_mean": float(np.mean([r["fp64_steps"] for r in reps])),
                        "node_steps": node_steps},
         "node_steps_per_s": node_steps / (kernel_ms * 1e-3),
+        "graph_replay": graph,
         "batched_fp64": fp64,
         "plain_fp64": plain,
         "speedup_vs_plain": (value / plain["networks_per_s"]) if plain else None,
@@ -328,6 +333,43 @@ def main():
     if args.calibrate_copy:  # after the timed work, for the counter passes
         out["copy_calibration"] = calibrate_copy(args.calibrate_copy)
     print(json.dumps(out), flush=True)
+
+
+def graph_leg(N, nets, specs, prm, args, shape):
+    """The batch as its workload states it: shapes from the autotuner
+    (nr_autotune: a probe relax per candidate shape and team rule, step
+    budgets proportional to each network's expected steps, 400 on average)
+    and the strain step captured as a HIP graph (nr_batch_graph_capture) and replayed
+    (nr_batch_graph_replay).  Two eager relaxes first (the second planned from
+    the first's step counts, as the timed headline relaxes are), then the
+    capture of the same step and one timed replay from the unstrained state;
+    the replay must reproduce the eager relax bit for bit."""
+    t0 = time.perf_counter()
+    tuned, predicted = N.autotune(nets, 1.0, 1.0, precision=N.PREC_MIXED, probe_steps=400)
+    t_tune = time.perf_counter() - t0
+    progress(f"graph leg: autotune chose {tuned} (predicted {predicted:.0f} ms) in {t_tune:.1f} s")
+    gb = N.Batch(nets, 1.0, 1.0, precision=N.PREC_MIXED, mixed_switch=args.switch, **tuned)
+    xd = np.array([s["gamma"] * n.info().height for s, n in zip(specs, nets)])
+    eager = None
+    for _ in range(2):
+        gb.reset()
+        gb.strain_all(xd)
+        eager = gb.relax(prm)
+    eager_ms = gb.last_ms()
+    gb.graph_capture(xd, 0.0, prm)
+    gb.reset()
+    reps = gb.graph_replay()
+    ms = gb.last_ms()
+    same = all((a["steps"], a["es"], a["eb"]) == (b["steps"], b["es"], b["eb"])
+               for a, b in zip(eager, reps))
+    n_conv = int(sum(r["status"] == N.CONVERGED and r["fmax"] < args.fcut for r in reps))
+    progress(f"graph leg: replay {ms:.0f} ms (eager {eager_ms:.0f} ms), same as eager: {same}")
+    return {"networks_per_s": n_conv / (ms * 1e-3), "replay_ms": ms, "eager_ms": eager_ms,
+            "converged": n_conv, "replay_equals_eager": bool(same),
+            "autotune": {"shape": tuned, "predicted_ms": predicted, "seconds": t_tune},
+            "note": "one strain step of every network captured as a HIP graph and replayed "
+                    "from the unstrained state, shapes from nr_autotune; device time of the "
+                    "replay (the graph's own copies of the reports included)"}
 
 
 def single_leg(N, args):

@@ -284,15 +284,15 @@ void nr_batch_destroy(nr_batch *b);
 
 /* Shape tuner (no counterpart in the reference, whose grid and block come
  * from the -x/-y flags, large_strain_relax.cu:1059-1063): for these networks
- * picks the workgroup/tile shape and the tiles-per-member rule.  The team
- * rule comes from a cost model of the persistent launch; each candidate
- * shape is timed by a probe relax of probe_steps FIRE steps per network.
- * opts is read (precision, device, team; a tiles_per_member > 0 is kept) and
- * written with the choice; *predicted_ms (may be NULL) estimates a relax of
- * the expected step counts (a fit over network size and bond density).
- * Results depend on the shape only
- * through reduction order, so use one choice for every batch whose results
- * are compared. */
+ * picks the workgroup/tile shape and the tiles-per-member rule (1, 2, 4 or 8,
+ * or opts->tiles_per_member when > 0) by timing each candidate with a probe
+ * relax: network k takes a step budget proportional to its expected step
+ * count (a fit over network size and bond density; the budgets average
+ * probe_steps), with no convergence test.  opts is read (precision, device,
+ * team) and written with the fastest choice; *predicted_ms (may be NULL) is
+ * its probe time scaled to the expected step counts.  Results depend on the
+ * shape only through reduction order, so use one choice for every batch
+ * whose results are compared. */
 int nr_autotune(int count, nr_network *const *nets, const double *ks,
                 const double *kb, int probe_steps, nr_batch_opts *opts,
                 double *predicted_ms);

@@ -318,6 +318,12 @@ class Batch:
             yr = i.height if i.height > 0 else i.maxy - i.miny
             _check(lib().nr_batch_strain(self.h, k, gamma * yr, 0.0))
 
+    def strain_all(self, xdisp, ydisp=None):
+        """Per-network affine steps (nr_batch_strain for each network)."""
+        for k in range(self.count):
+            _check(lib().nr_batch_strain(self.h, k, float(xdisp[k]),
+                                         0.0 if ydisp is None else float(ydisp[k])))
+
     def relax(self, params):
         reps = (Report * self.count)()
         if isinstance(params, RelaxParams):

@@ -950,11 +950,15 @@ int nr_batch_counts(const nr_batch *b, int index, int64_t *nodes, int64_t *bonds
 
 void nr_batch_destroy(nr_batch *b) { delete b; }
 
-// Shape tuner.  For each candidate workgroup/tile shape the cost model picks
-// the tiles-per-member rule (teams.cpp), then a probe relax of probe_steps
-// FIRE steps per network (no convergence test) measures that shape's cost
-// per model unit on this GPU; the shape with the lowest predicted time for
-// a relax of the expected step counts (teams.hpp) wins.
+// Shape tuner.  Every candidate workgroup/tile shape and tiles-per-member
+// rule ({1, 2, 4, 8}, or the caller's) is measured by a probe relax in which
+// network k takes a step budget proportional to its expected step count (a
+// fit over size and bond density; the budgets average probe_steps), teams
+// planned from those budgets as a relax planned from known step counts is,
+// and no convergence test.  The fastest candidate wins; its probe time scaled
+// by the expected steps over the budgets is the prediction.  (Extrapolating
+// a uniform probe with the makespan model ranked large teams too well: at 16
+// tiles per member it predicted 6.7 s for a batch that took 10.3 s.)
 int nr_autotune(int count, nr_network *const *nets, const double *ks, const double *kb,
                 int probe_steps, nr_batch_opts *opts, double *predicted_ms)
 {
@@ -966,54 +970,59 @@ int nr_autotune(int count, nr_network *const *nets, const double *ks, const doub
         };
         std::vector<Shape> shapes = {{256, 256}, {512, 512}};
         if (opts->precision == NR_PREC_FP64) shapes.push_back({256, 512});
-        const std::vector<int> cands = {1, 2, 3, 4, 5, 6, 8, 12, 16};
+        std::vector<int> cands = {1, 2, 4, kDefaultTilesPerMember};
+        if (opts->tiles_per_member > 0) cands = {opts->tiles_per_member};
+        nr_relax_params base;
+        std::memset(&base, 0, sizeof(base));
+        base.alpha0 = 0.1;
+        base.f_alpha = 0.99;
+        base.dt_max = 0.1;
+        base.f_inc = 1.1;
+        base.f_dec = 0.5;
+        base.nmin = 5;
+        base.mass = 1.0;
+        base.fcut = 0.0;  // never met: every network takes its whole budget
+        base.criterion = NR_CRIT_MAX;
+        base.method = NR_METHOD_FIRE;
         double best = std::numeric_limits<double>::infinity();
         nr_batch_opts best_opts = *opts;
         for (const Shape &sh : shapes) {
             nr_batch_opts o = *opts;
             o.block = sh.block;
             o.tile_nodes = sh.tile;
-            o.tiles_per_member = 1;
+            o.tiles_per_member = cands[0];
             nr_batch *raw = nullptr;
             int rc = nr_batch_create(count, nets, ks, kb, &o, &raw);
             if (rc == NR_E_TOPO || rc == NR_E_ARG) continue;  // shape cannot hold these networks
             if (rc) return rc;
             std::unique_ptr<nr_batch> b(raw);
-            std::vector<int> tiles(count);
-            std::vector<double> full(count), probe(count, (double)probe_steps);
+            std::vector<double> full(count), budget(count);
+            double sum_full = 0.0;
             for (int k = 0; k < count; k++) {
-                tiles[k] = b->plan.nets[k].n_tiles;
                 full[k] = nr::expected_steps(b->plan.nets[k].n_nodes, b->plan.nets[k].n_bonds);
+                sum_full += full[k];
             }
-            const int tpm = opts->tiles_per_member > 0
-                                ? opts->tiles_per_member
-                                : nr::tune_tiles_per_member(tiles, full, b->team_cap, b->resident,
-                                                            cands, nullptr);
-            rc = set_team_layout(b.get(), tpm);
-            if (rc) return rc;
-            nr_relax_params prm;
-            std::memset(&prm, 0, sizeof(prm));
-            prm.alpha0 = 0.1;
-            prm.f_alpha = 0.99;
-            prm.dt_max = 0.1;
-            prm.f_inc = 1.1;
-            prm.f_dec = 0.5;
-            prm.nmin = 5;
-            prm.mass = 1.0;
-            prm.fcut = 0.0;  // never met: every network takes probe_steps steps
-            prm.max_steps = probe_steps;
-            prm.criterion = NR_CRIT_MAX;
-            prm.method = NR_METHOD_FIRE;
-            rc = nr_batch_relax(b.get(), &prm, 1, nullptr);
-            if (rc) return rc;
-            const double units_probe =
-                nr::predict_time(tiles, probe, b->team_cap, b->resident, tpm);
-            const double units_full = nr::predict_time(tiles, full, b->team_cap, b->resident, tpm);
-            const double t = b->last_ms / units_probe * units_full;
-            if (t < best) {
-                best = t;
-                best_opts = o;
-                best_opts.tiles_per_member = tpm;
+            const double frac = probe_steps * (double)count / sum_full;
+            double sum_budget = 0.0;
+            std::vector<nr_relax_params> prm(count, base);
+            for (int k = 0; k < count; k++) {
+                budget[k] = std::max(1.0, std::round(full[k] * frac));
+                prm[k].max_steps = (int)budget[k];
+                sum_budget += budget[k];
+            }
+            for (int tpm : cands) {
+                b->hist_steps = budget;
+                rc = set_team_layout(b.get(), tpm);
+                if (rc) return rc;
+                b->hist_planned = true;
+                rc = nr_batch_relax(b.get(), prm.data(), 0, nullptr);
+                if (rc) return rc;
+                const double t = b->last_ms * sum_full / sum_budget;
+                if (t < best) {
+                    best = t;
+                    best_opts = o;
+                    best_opts.tiles_per_member = tpm;
+                }
             }
         }
         if (!std::isfinite(best)) return fail(NR_E_TOPO, "no shape holds these networks");
