@@ -368,7 +368,8 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
         HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
         if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
         b->resident = per_cu * n_cu;
-        b->team_cap = std::min(opts && opts->team > 0 ? opts->team : kDefaultTeamCap, b->resident);
+        b->team_cap = std::min({opts && opts->team > 0 ? opts->team : kDefaultTeamCap, b->resident,
+                                nr::max_team(b->block, b->npt)});
         rc = set_team_layout(b.get(), opts && opts->tiles_per_member > 0 ? opts->tiles_per_member
                                                                    : tuned_tiles_per_member(b.get()));
         if (rc) return rc;

@@ -118,6 +118,12 @@ inline int lds_per_workgroup(int block) { return block == 256 ? 80 * 1024 : 156 
 // Partial-sum record of a team member: two step parities x 8 words, each
 // word = step tag (high 32 bits) | one half of a double (low 32 bits).
 constexpr int kPartWords = 16;
+// Every member reads every member's record, four lanes per record, at most
+// poll_sweeps(npt) records per lane (fewer in the kernels with two nodes per
+// thread, which have fewer registers to spare): a team has at most
+// poll_sweeps(npt) * block / 4 members.
+constexpr int poll_sweeps(int npt) { return npt == 1 ? 8 : 4; }
+inline int max_team(int block, int npt) { return poll_sweeps(npt) * block / 4; }
 
 // Team barrier words per team: [0] root arrivals, [16] generation,
 // [32 + 16 g] arrivals of group g (groups of kBarGroup members).

@@ -195,41 +195,6 @@ __device__ inline void put_record(unsigned long long *w, unsigned tag, const Red
     }
 }
 
-__device__ inline double join_halves(unsigned long long hi, unsigned long long lo)
-{
-    return __longlong_as_double((long long)(((hi & 0xFFFFFFFFull) << 32) | (lo & 0xFFFFFFFFull)));
-}
-
-// Waits (bounded) until the record at w carries `tag`; false when the wait
-// bound expired or another workgroup reported an expired wait.
-__device__ bool poll_record(const KArgs &A, const unsigned long long *w, unsigned tag, Red4 &v)
-{
-    const long long t0 = wall_clock64();
-    for (;;) {
-        unsigned long long x[8];
-#pragma unroll
-        for (int i = 0; i < 8; i++)
-            x[i] = __hip_atomic_load(const_cast<unsigned long long *>(w + i), __ATOMIC_RELAXED,
-                                     __HIP_MEMORY_SCOPE_AGENT);
-        bool ok = true;
-#pragma unroll
-        for (int i = 0; i < 8; i++) ok = ok && (unsigned)(x[i] >> 32) == tag;
-        if (ok) {
-            v.a = join_halves(x[0], x[1]);
-            v.b = join_halves(x[2], x[3]);
-            v.c = join_halves(x[4], x[5]);
-            v.m = join_halves(x[6], x[7]);
-            return true;
-        }
-        if (wall_clock64() - t0 > A.timeout_ticks ||
-            __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) {
-            __hip_atomic_store(A.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            return false;
-        }
-        __builtin_amdgcn_s_sleep(1);
-    }
-}
-
 // Bounded two-level team barrier, executed by thread 0 of each member after
 // its workgroup drained its stores.  Arrival counters only grow: the member
 // that brings its group's count to a multiple of the group size arrives at
@@ -283,8 +248,6 @@ __device__ bool team_sync(const KArgs &A, Team &T, int *s_flag)
     return *s_flag != 0;
 }
 
-// Block + team reduction of a Red4 in a fixed order (wave butterfly, waves
-// in order, members in order).  Every thread gets the result.
 // First triple of thread tid in the element phase: triple chunk c (64
 // consecutive triples) goes to wave (c + bond chunks) mod (B / 64).
 template <int B>
@@ -295,7 +258,11 @@ __device__ inline int tri_first(int tid, int nb)
     return ((wave - nbc % W + W) % W) * 64 + (tid & 63);
 }
 
-template <int B>
+// Block + team reduction of a Red4 in a fixed order (wave butterfly, waves
+// in order; in a team, each member's block total is published as a tagged
+// record and every member combines all records in the same fixed order).
+// Every thread gets the result.
+template <int B, int KS>
 __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_red, int *s_flag)
 {
     constexpr int W = B / 64;
@@ -335,35 +302,80 @@ __device__ bool reduce4(const KArgs &A, Team &T, Red4 r, Red4 &out, double *s_re
     }
     if (team) {
         __syncthreads();
-        // members' partials, one per thread, reduced by the same fixed tree
-        Red4 t = {0.0, 0.0, 0.0, 0.0};
-        for (int m0 = 0; m0 < T.g; m0 += B) {
-            const int m = m0 + (int)threadIdx.x;
-            Red4 v = {0.0, 0.0, 0.0, 0.0};
-            if (m < T.g &&
-                !poll_record(A, A.partials + (size_t)(T.part + m) * kPartWords + (tag & 1) * 8, tag, v))
-                *s_flag = 0;
-            v.a = wave_sum(v.a);
-            v.b = wave_sum(v.b);
-            v.c = wave_sum(v.c);
-            v.m = wave_max(v.m);
-            __syncthreads();
-            if (lane == 0) {
-                s_red[4 * wave + 0] = v.a;
-                s_red[4 * wave + 1] = v.b;
-                s_red[4 * wave + 2] = v.c;
-                s_red[4 * wave + 3] = v.m;
+        // Every member reads every member's record.  Four lanes share one:
+        // lane q loads words 2q, 2q+1 (double q) in one 16-byte load, so a
+        // wave load fetches 16 whole records and a spin costs one request
+        // per record line.  Lane (r, q) polls records r, r + B/4, ... in two
+        // groups of loads, and sums them in that order; lanes of equal q
+        // then combine by butterfly, and waves in order.
+        constexpr int R = B / 4;
+        const int q = threadIdx.x & 3, rr = threadIdx.x >> 2;
+        const __amdgpu_buffer_rsrc_t rs = __builtin_amdgcn_make_buffer_rsrc(
+            A.partials + (size_t)T.part * kPartWords, (short)0, T.g * kPartWords * 8, kBufWord3);
+        const int boff = (int)(tag & 1) * 64 + q * 16;
+        double got[KS];
+        unsigned pending = 0;
+#pragma unroll
+        for (int k = 0; k < KS; k++) {
+            got[k] = 0.0;
+            if (rr + k * R < T.g) pending |= 1u << k;
+        }
+        bool ok = T.g <= KS * R;  // the host caps teams at max_team(B, NPT)
+        if (!ok) pending = 0;
+        const long long t0 = wall_clock64();
+        int spins = 0;
+        while (pending) {
+#pragma unroll
+            for (int k0 = 0; k0 < KS; k0 += 4) {
+                u32x4 w[4];
+#pragma unroll
+                for (int k = k0; k < k0 + 4; k++)
+                    if (pending & (1u << k))  // volatile (bit 31): reloaded every spin
+                        w[k - k0] = __builtin_amdgcn_raw_buffer_load_b128(
+                            rs, (rr + k * R) * (kPartWords * 8) + boff, 0,
+                            kPolicySc1 | (int)0x80000000);
+#pragma unroll
+                for (int k = k0; k < k0 + 4; k++)
+                    if ((pending & (1u << k)) && w[k - k0].y == tag && w[k - k0].w == tag) {
+                        got[k] = __longlong_as_double(
+                            (long long)(((unsigned long long)w[k - k0].x << 32) | w[k - k0].z));
+                        pending &= ~(1u << k);
+                    }
             }
-            __syncthreads();
+            if (!pending) break;
+            // the wait bound and the error flag are checked every 64 spins
+            if ((++spins & 63) == 0 &&
+                (wall_clock64() - t0 > A.timeout_ticks ||
+                 __hip_atomic_load(A.err, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0)) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (!ok) {
+            __hip_atomic_store(A.err, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            *s_flag = 0;
+        }
+        double v = 0.0;
+#pragma unroll
+        for (int k = 0; k < KS; k++)
+            if (rr + k * R < T.g) v = q == 3 ? fmax(v, got[k]) : v + got[k];
+#pragma unroll
+        for (int o = 4; o < 64; o <<= 1) {
+            const double u = __shfl_xor(v, o, 64);
+            v = q == 3 ? fmax(v, u) : v + u;
+        }
+        __syncthreads();
+        if (lane < 4) s_red[4 * wave + lane] = v;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            Red4 t = {0.0, 0.0, 0.0, 0.0};
             for (int w = 0; w < W; w++) {
                 t.a += s_red[4 * w + 0];
                 t.b += s_red[4 * w + 1];
                 t.c += s_red[4 * w + 2];
                 t.m = fmax(t.m, s_red[4 * w + 3]);
             }
-        }
-        __syncthreads();
-        if (threadIdx.x == 0) {
             s_red[4 * W + 0] = t.a;
             s_red[4 * W + 1] = t.b;
             s_red[4 * W + 2] = t.c;
@@ -1298,7 +1310,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             int since_best;
             if (fresh) {
                 part = pass32<B, kPass32Init>(A, T, C, S, p, pa, L, resident);
-                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+                if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
                 S.s = 1;
                 S.pend_x = 0.0;
                 S.pend_y = 0.0;
@@ -1319,7 +1331,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 used++;
                 fire.load(S);
                 part = pass32<B, kPass32Step>(A, T, C, S, p, pa, L, resident);
-                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
+                if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) {
                     status = NR_ABORTED;
                     break;
                 }
@@ -1345,7 +1357,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 steps32 = step;
                 p = pa;
                 part = pass<B, NPT, kPassFuse>(A, T, C, S, p, L, resident);
-                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+                if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
                 p ^= 1;
                 ff = r.c;
                 fmax2 = r.m;
@@ -1357,7 +1369,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             }
         } else if (fresh) {
             part = pass<B, NPT, kPassInit>(A, T, C, S, p, L, resident);
-            if (!reduce4<B>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
+            if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
             S.pend_x = 0.0;
             S.pend_y = 0.0;
             p ^= 1;
@@ -1419,7 +1431,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                     S.cg_a = a;
                     S.cg_first = ls == 0;
                     part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L, resident);
-                    if (!reduce4<B>(A, T, part, tr, s_red, s_flag)) {
+                    if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, tr, s_red, s_flag)) {
                         status = NR_ABORTED;
                         break;
                     }
@@ -1437,7 +1449,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                     S.cg_a = a;
                     S.cg_first = 0;
                     part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L, resident);
-                    if (!reduce4<B>(A, T, part, tr, s_red, s_flag)) {
+                    if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, tr, s_red, s_flag)) {
                         status = NR_ABORTED;
                         break;
                     }
@@ -1477,7 +1489,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 used++;
                 fire.load(S);
                 part = pass<B, NPT, kPassStep>(A, T, C, S, p, L, resident);
-                if (!reduce4<B>(A, T, part, r, s_red, s_flag)) {
+                if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) {
                     status = NR_ABORTED;
                     break;
                 }
@@ -1508,7 +1520,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
         Red4 e = {0.0, 0.0, 0.0, 0.0};
         if (status != NR_ABORTED) {
             part = pass<B, NPT, kPassEnergy>(A, T, C, S, p, L, resident);
-            if (!reduce4<B>(A, T, part, e, s_red, s_flag)) status = NR_ABORTED;
+            if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, e, s_red, s_flag)) status = NR_ABORTED;
         }
         if (lead) {
             NetOut o;

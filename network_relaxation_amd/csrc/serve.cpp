@@ -14,6 +14,7 @@
 // result is the same alone and in any mix.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -647,7 +648,8 @@ int nr_server_create(const nr_batch_opts *shape, const nr_server_opts *opts, nr_
         HIP_TRY((hipError_t)nr::max_resident_blocks(s->block, s->npt, s->lds, &per_cu, &n_cu));
         if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
         s->resident = per_cu * n_cu;
-        s->team_cap = std::min(shape && shape->team > 0 ? shape->team : 512, s->resident);
+        s->team_cap = std::min({shape && shape->team > 0 ? shape->team : 512, s->resident,
+                                nr::max_team(s->block, s->npt)});
 
         // arena: per node the FP64 state (x, v, F in two parities), its mask, the
         // FP32 stage state when mixed, and about 88 B of tile topology
