@@ -538,6 +538,30 @@ __device__ inline double2 advance(const double2 x, const double2 v, const double
     return xn;
 }
 
+// 1/sqrt(x) and 1/x for the force terms: the hardware estimate refined by
+// two Newton steps (y += y (1 - x y^2) / 2, y += y (1 - x y)), each of which
+// squares the relative error of the estimate, so the result is within an ulp
+// or so of the correctly rounded value; x = 0 gives a non-finite force as the
+// correctly rounded forms do.
+__device__ inline double rsqrt_newton(double x)
+{
+    double y = __builtin_amdgcn_rsq(x);
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+        const double r = fma(-(x * y), y, 1.0);
+        y = fma(0.5 * y, r, y);
+    }
+    return y;
+}
+
+__device__ inline double rcp_newton(double x)
+{
+    double y = __builtin_amdgcn_rcp(x);
+#pragma unroll
+    for (int i = 0; i < 2; i++) y = fma(y, fma(-x, y, 1.0), y);
+    return y;
+}
+
 // Bond vector r = x_j + image - x_i of bond record e (axial_forces :401-404,
 // written as x_j + offset - x_i).
 __device__ inline void bond_vec(const Blob &bb, const double2 *pos, const NetCtx &C, int e,
@@ -1108,7 +1132,7 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
             } else {
                 // :401-404: f = ks (1 - L/|r|) r on the first node
                 const double d2 = fma(rx, rx, ry * ry);
-                const double inv = 1.0 / sqrt(d2);
+                const double inv = rsqrt_newton(d2);
                 const double s = C.ks * (1.0 - bb.bl[e] * inv);
                 L.fv[e] = make_double2(s * rx, s * ry);
             }
@@ -1135,8 +1159,8 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
                 }
             } else {
                 // :453-468 force on end a, end b and the hinge
-                const double ia = 1.0 / fma(ax, ax, ay * ay);
-                const double ib = 1.0 / fma(bx, bx, by * by);
+                const double ia = rcp_newton(fma(ax, ax, ay * ay));
+                const double ib = rcp_newton(fma(bx, bx, by * by));
                 const double g = kk * dth;
                 double f2x = g * ia;
                 const double f2y = f2x * ax;
