@@ -1,8 +1,8 @@
 """GPU parity: the HIP relax path against the FP64 oracle, through the C ABI.
 
-Tolerances: forces agree to 1e-12 of the largest force (bond terms are
-bit-identical by construction; triple terms differ only by the last bits of
-atan2); relaxed energies agree to 1e-12 relative at |F|max < 1e-10 when both
+Tolerances: forces agree to 1e-12 of the largest force (the device's 1/|r|,
+1/|a|^2 and atan2 differ from the oracle's only in the last bits); relaxed
+energies agree to 1e-12 relative at |F|max < 1e-10 when both
 minimisers stay in one basin (small strain steps, as the reference's
 protocol takes them).
 """
