@@ -594,20 +594,22 @@ def served_leg(N, args):
     return out
 
 
-# HBM bytes per composite FIRE step at L = 512 (FETCH_SIZE x 2 + WRITE_SIZE,
-# KiB x 1024, summed over the step's three kernels: profiles/mesh_kernels_composite512.md)
-COMPOSITE512_TRAFFIC_BYTES = ((34216 * 2 + 40352) + (34864 * 2 + 8201) + (6124 * 2 + 8135)) * 1024
+# HBM bytes per composite FIRE step at L = 512 from the kernels' request
+# counters (32 RDREQ_32B + 64 RDREQ_64B + 128 RDREQ_128B read, 64 WRREQ_64B +
+# 32 (WRREQ - WRREQ_64B) written; profiles/mesh_kernels_composite512.md):
+# mesh_tile, then mesh_fire.
+COMPOSITE512_TRAFFIC_BYTES = ((64 * 2258 + 128 * 678279 + 64 * 221311 + 32 * (223848 - 221311)) +
+                              (64 * 423 + 128 * 97559 + 64 * 130161 + 32 * (130162 - 130161)))
 
 
 def composite_bytes(n, e, t, f):
-    """Algorithmic bytes of one composite FIRE step (three launches):
-    mesh_elements reads topology + positions and writes slots (bond 72 B,
-    triple 112 B, facet 168 B); mesh_gather reads each slot entry (4 B index
-    + 16 B slot; 2 per bond, 4 per triple, 3 per facet), row, clamp code and
-    twin index (9 B per point) and v, writes F and v (48 B per point);
+    """Algorithmic bytes of one composite FIRE step (two launches): the tile
+    pass reads each element's record once (bond: 2 point indices, rest length
+    and x image offset, 24 B; triple: 3 indices and 2 offsets, 28 B; facet: 3
+    indices, the inverse rest matrix and the area, 68 B), each point's position
+    (16 B), velocity (16 B) and clamp code (1 B) and writes its F and v (32 B);
     mesh_fire reads v, F, x and writes v, x (80 B per point)."""
-    entries = 2 * e + 4 * t + 3 * f
-    return 72 * e + 112 * t + 168 * f + 20 * entries + n * (9 + 48 + 80)
+    return 24 * e + 28 * t + 68 * f + n * (16 + 16 + 1 + 32 + 80)
 
 
 def composite_leg(N, args):
@@ -648,9 +650,9 @@ def composite_leg(N, args):
                         "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                         "traffic": COMPOSITE512_TRAFFIC_BYTES if L == 512 else None,
                         "alg_bytes_per_step": step_bytes,
-                        "note": "three launches per FIRE step; achieved = algorithmic bytes "
-                                "of the step x steps / device time; traffic = counted HBM "
-                                "bytes per step"}}
+                        "note": "two launches per FIRE step (fused element + gather tile "
+                                "pass, FIRE update); achieved = algorithmic bytes of the step "
+                                "x steps / device time; traffic = counted HBM bytes per step"}}
     if not getattr(args, "no_cpu", False):
         try:
             import oracle as O

@@ -4,12 +4,16 @@
 // :1095-1391, run_fire_md :865-1066).
 //
 // A relax uploads the FIRE scalars, zeroes v, computes the first force and
-// drift, then replays a captured chunk of kMeshChunk steps (three kernels
-// each, mesh_kernels.hip) until the device's done flag is up; positions stay
-// resident between the increments of a ramp.  With NRELAX_GRAPHS=0 in the
-// environment at nr_composite_create, each chunk's kernels are launched
-// directly instead (same kernels, same order; for tools that do not follow
-// graph launches).
+// drift, then replays a captured chunk of kMeshChunk steps (two kernels
+// each, mesh_kernels.hip: the fused element + gather pass over spatial tiles
+// and the FIRE update) until the device's done flag is up; positions stay
+// resident between the increments of a ramp.  Points live on the device in
+// Morton order (positions and forces are permuted at the C ABI).  With
+// NRELAX_GRAPHS=0 in the environment at nr_composite_create, each chunk's
+// kernels are launched directly instead (same kernels, same order; for tools
+// that do not follow graph launches); with NRELAX_MESH_TILES=0 a step runs
+// the unfused element and gather kernels (three launches, the same forces
+// bit for bit).
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -37,6 +41,9 @@ struct nr_mesh {
 namespace {
 
 constexpr int kMeshChunk = 32;  // FIRE steps per captured graph (even)
+// Dynamic LDS of a composite tile: two workgroups of mesh_tile share a CU's
+// 160 KB (with their static LDS).
+constexpr int kMeshTileLds = 76 * 1024;
 
 }  // namespace
 
@@ -54,6 +61,14 @@ struct nr_composite {
     nr::DevBuf<int4> tri, facet;
     nr::DevBuf<int> ent, cnt, twin;
     nr::DevBuf<unsigned char> fcode;
+    std::vector<int> dev_of;  // device index of each point of the file
+    // fused element + gather kernel
+    nr::DevBuf<nr::MeshTile> tiles;
+    nr::DevBuf<uint4> tblob;
+    nr::DevBuf<unsigned> tb_ij;
+    nr::DevBuf<double2> tb_geo, tt_off, tf_m;
+    nr::DevBuf<uint2> tt_idx, tf_idx;
+    nr::DevBuf<double> tf_area;
     nr::MeshDev md{};
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
@@ -127,6 +142,246 @@ int build_gather(int n, const std::vector<int> &bonds, int nb, const std::vector
     return NR_OK;
 }
 
+// Host side of the fused element + gather kernel: tiles of MeshTile with
+// their blobs and tile-major element records.
+struct TileArrays {
+    std::vector<MeshTile> tiles;
+    std::vector<uint8_t> blob;
+    std::vector<unsigned> b_ij;
+    std::vector<double2> b_geo;
+    std::vector<uint2> t_idx;
+    std::vector<double2> t_off;
+    std::vector<uint2> f_idx;
+    std::vector<double2> f_m;
+    std::vector<double> f_area;
+    int max_lds = 0;
+};
+
+uint32_t spread_bits(uint32_t x)
+{
+    x &= 0xFFFF;
+    x = (x | (x << 8)) & 0x00FF00FF;
+    x = (x | (x << 4)) & 0x0F0F0F0F;
+    x = (x | (x << 2)) & 0x33333333;
+    x = (x | (x << 1)) & 0x55555555;
+    return x;
+}
+
+// Device order of the composite's points: Morton order of their positions
+// (16 bits per axis, ties by index), so that a tile's points are one
+// contiguous run of the state arrays and its neighbours lie close by.
+std::vector<int> morton_order(int n, const std::vector<double> &pos)
+{
+    std::vector<int> order((size_t)n);
+    if (n <= 0) return order;
+    double x0 = pos[0], x1 = pos[0], y0 = pos[1], y1 = pos[1];
+    for (int i = 0; i < n; i++) {
+        x0 = std::min(x0, pos[2 * i]);
+        x1 = std::max(x1, pos[2 * i]);
+        y0 = std::min(y0, pos[2 * i + 1]);
+        y1 = std::max(y1, pos[2 * i + 1]);
+    }
+    const double sx = x1 > x0 ? 65535.0 / (x1 - x0) : 0.0, sy = y1 > y0 ? 65535.0 / (y1 - y0) : 0.0;
+    std::vector<std::pair<uint32_t, int>> key((size_t)n);
+    for (int i = 0; i < n; i++) {
+        const uint32_t qx = (uint32_t)((pos[2 * i] - x0) * sx);
+        const uint32_t qy = (uint32_t)((pos[2 * i + 1] - y0) * sy);
+        key[i] = {spread_bits(qx) | spread_bits(qy) << 1, i};
+    }
+    std::sort(key.begin(), key.end());
+    for (int i = 0; i < n; i++) order[i] = key[i].second;
+    return order;
+}
+
+// Points (in device order) cut into runs of tile_points (halved until a
+// tile's LDS fits lds_budget and its elements the registers of mesh_tile).
+// A tile takes
+// every element incident to an own point or to its twin (twin non-empty:
+// coupled twins), in global element order per type; its points are the own
+// ones, then the others those elements reach in order of first use; each own
+// point's entries follow build_gather's order (bonds, triples, facets, each
+// in element order), then its twin's.
+int build_tiles(int n, const std::vector<int> &bonds,
+                const std::vector<double2> &geo, int nb, const std::vector<int> &tri,
+                const std::vector<double2> &toff, int nt, const std::vector<int> &facets,
+                const std::vector<double> &inv, const std::vector<double> &area, int nf,
+                const std::vector<int> &twin, const std::vector<unsigned char> &fcode,
+                int lds_budget, int tile_points, TileArrays &T)
+{
+    T = TileArrays();
+    if (n <= 0) return NR_OK;
+    // incident elements per point, in element order: bond 2b | second,
+    // triple 3t + role (0 hinge, 1 end a, 2 end b), facet 3f + corner
+    std::vector<std::vector<int>> binc((size_t)n), tinc((size_t)n), finc((size_t)n);
+    for (int b = 0; b < nb; b++) {
+        binc[bonds[2 * b]].push_back(2 * b);
+        binc[bonds[2 * b + 1]].push_back(2 * b + 1);
+    }
+    for (int t = 0; t < nt; t++)
+        for (int r = 0; r < 3; r++) tinc[tri[3 * t + r]].push_back(3 * t + r);
+    for (int f = 0; f < nf; f++)
+        for (int k = 0; k < 3; k++) finc[facets[3 * f + k]].push_back(3 * f + k);
+    std::vector<int> lstamp((size_t)n, -1), loc((size_t)n, 0);
+    std::vector<int> bstamp((size_t)nb, -1), tstamp((size_t)nt, -1), fstamp((size_t)nf, -1);
+    std::vector<int> bl((size_t)nb, 0), tl((size_t)nt, 0), fl((size_t)nf, 0);
+    int stamp = 0;
+    // ranges of the Morton order, in order; one that does not fit is halved
+    // in place
+    std::vector<std::pair<int, int>> ranges;
+    for (int a = 0; a < n; a += tile_points) ranges.push_back({a, std::min(n, a + tile_points)});
+    std::vector<int> own, halo, blist, tlist, flist, ent;
+    std::vector<uint16_t> ioff, imid;
+    std::vector<uint8_t> mask;
+    for (size_t ri = 0; ri < ranges.size(); ri++) {
+        const int a = ranges[ri].first, e = ranges[ri].second;
+        const int tag = stamp++;
+        own.clear();
+        halo.clear();
+        blist.clear();
+        tlist.clear();
+        flist.clear();
+        for (int k = a; k < e; k++) {
+            const int p = k;
+            own.push_back(p);
+            lstamp[p] = tag;
+            loc[p] = k - a;
+        }
+        auto collect = [&](int q) {
+            for (int c : binc[q])
+                if (bstamp[c >> 1] != tag) {
+                    bstamp[c >> 1] = tag;
+                    blist.push_back(c >> 1);
+                }
+            for (int c : tinc[q])
+                if (tstamp[c / 3] != tag) {
+                    tstamp[c / 3] = tag;
+                    tlist.push_back(c / 3);
+                }
+            for (int c : finc[q])
+                if (fstamp[c / 3] != tag) {
+                    fstamp[c / 3] = tag;
+                    flist.push_back(c / 3);
+                }
+        };
+        for (int p : own) {
+            collect(p);
+            if (!twin.empty() && twin[p] >= 0) collect(twin[p]);
+        }
+        std::sort(blist.begin(), blist.end());
+        std::sort(tlist.begin(), tlist.end());
+        std::sort(flist.begin(), flist.end());
+        auto reach = [&](int q) {
+            if (lstamp[q] != tag) {
+                lstamp[q] = tag;
+                loc[q] = (int)(own.size() + halo.size());
+                halo.push_back(q);
+            }
+        };
+        for (size_t i = 0; i < blist.size(); i++) {
+            bl[blist[i]] = (int)i;
+            reach(bonds[2 * blist[i]]);
+            reach(bonds[2 * blist[i] + 1]);
+        }
+        for (size_t i = 0; i < tlist.size(); i++) {
+            tl[tlist[i]] = (int)i;
+            for (int r = 0; r < 3; r++) reach(tri[3 * tlist[i] + r]);
+        }
+        for (size_t i = 0; i < flist.size(); i++) {
+            fl[flist[i]] = (int)i;
+            for (int k = 0; k < 3; k++) reach(facets[3 * flist[i] + k]);
+        }
+        const int n_own = (int)own.size(), n_loc = n_own + (int)halo.size();
+        const int tb = (int)blist.size(), tt = (int)tlist.size(), tf = (int)flist.size();
+        const long n_fv = (long)tb + 2L * tt + 3L * tf;
+        auto entries = [&](int q) {
+            for (int c : binc[q]) ent.push_back(bl[c >> 1] << 1 | (c & 1));
+            for (int c : tinc[q]) {
+                const int t = tl[c / 3], role = c % 3;
+                if (role == 0) {
+                    ent.push_back((tb + t) << 1 | 1);
+                    ent.push_back((tb + tt + t) << 1 | 1);
+                } else {
+                    ent.push_back((role == 1 ? tb + t : tb + tt + t) << 1);
+                }
+            }
+            for (int c : finc[q]) ent.push_back((tb + 2 * tt + (c % 3) * tf + fl[c / 3]) << 1);
+        };
+        ent.clear();
+        ioff.assign((size_t)n_own + 1, 0);
+        imid.assign((size_t)n_own, 0);
+        mask.assign((size_t)n_own, 0);
+        for (int k = 0; k < n_own; k++) {
+            const int p = own[k];
+            ioff[k] = (uint16_t)std::min<size_t>(ent.size(), 0xFFFF);
+            entries(p);
+            imid[k] = (uint16_t)std::min<size_t>(ent.size(), 0xFFFF);
+            mask[k] = (uint8_t)(fcode[p] & 3);
+            if (!twin.empty() && twin[p] >= 0) {
+                entries(twin[p]);
+                mask[k] |= (uint8_t)((fcode[twin[p]] & 3) << 2 | 16);
+            }
+        }
+        ioff[n_own] = (uint16_t)std::min<size_t>(ent.size(), 0xFFFF);
+        const MeshBlob B = mesh_blob(n_own, n_loc - n_own, (int)ent.size());
+        const long lds = (long)B.bytes + 16L * n_loc + 16L * n_fv;
+        const bool fits = lds <= lds_budget && n_loc <= kTileLocalPerThread * 256 && n_fv <= 32767 &&
+                          ent.size() <= 65535 && B.bytes <= kTileBlobRows * 256 * 16 &&
+                          tb <= kTileBondsPerThread * 256 &&
+                          tt <= kTileTriplesPerThread * 256 && tf <= kTileFacetsPerThread * 256;
+        if (!fits) {
+            if (e - a <= 8) return fail(NR_E_TOPO, "a composite tile exceeds the LDS budget");
+            // halve this range and redo it (the halves are processed next)
+            const int mid = a + (e - a) / 2;
+            ranges.insert(ranges.begin() + (long)ri + 1, {{a, mid}, {mid, e}});
+            continue;
+        }
+        MeshTile t{};
+        t.blob16 = (int)(T.blob.size() / 16);
+        t.blob_bytes = B.bytes;
+        t.n_own = n_own;
+        t.n_halo = n_loc - n_own;
+        t.n_inc = (int)ent.size();
+        t.nb = tb;
+        t.nt = tt;
+        t.nf = tf;
+        t.b0 = (int)T.b_ij.size();
+        t.t0 = (int)T.t_idx.size();
+        t.f0 = (int)T.f_idx.size();
+        T.tiles.push_back(t);
+        T.max_lds = std::max<int>(T.max_lds, (int)lds);
+        const size_t base = T.blob.size();
+        T.blob.resize(base + (size_t)B.bytes, 0);
+        uint8_t *bp = T.blob.data() + base;
+        int32_t *ids = reinterpret_cast<int32_t *>(bp + B.ids);
+        for (int k = 0; k < n_own; k++) ids[k] = own[k];
+        for (size_t k = 0; k < halo.size(); k++) ids[n_own + k] = halo[k];
+        std::memcpy(bp + B.ioff, ioff.data(), 2 * ioff.size());
+        std::memcpy(bp + B.imid, imid.data(), 2 * imid.size());
+        std::memcpy(bp + B.mask, mask.data(), mask.size());
+        uint16_t *ip = reinterpret_cast<uint16_t *>(bp + B.inc);
+        for (size_t k = 0; k < ent.size(); k++) ip[k] = (uint16_t)ent[k];
+        for (int b : blist) {
+            T.b_ij.push_back((unsigned)loc[bonds[2 * b]] | (unsigned)loc[bonds[2 * b + 1]] << 16);
+            T.b_geo.push_back(geo[b]);
+        }
+        for (int q : tlist) {
+            T.t_idx.push_back(make_uint2((unsigned)loc[tri[3 * q]] | (unsigned)loc[tri[3 * q + 1]] << 16,
+                                         (unsigned)loc[tri[3 * q + 2]]));
+            T.t_off.push_back(toff[q]);
+        }
+        for (int f : flist) {
+            T.f_idx.push_back(make_uint2(
+                (unsigned)loc[facets[3 * f]] | (unsigned)loc[facets[3 * f + 1]] << 16,
+                (unsigned)loc[facets[3 * f + 2]]));
+            for (int k = 0; k < 3; k++)
+                T.f_m.push_back(make_double2(inv[6 * (size_t)f + 2 * k], inv[6 * (size_t)f + 2 * k + 1]));
+            T.f_area.push_back(area[f]);
+        }
+    }
+    if (T.tiles.size() > (size_t)(1 << 30)) return fail(NR_E_TOPO, "too many composite tiles");
+    return NR_OK;
+}
+
 int composite_upload(nr_composite &c)
 {
     const Network &net = c.mesh.net;
@@ -136,39 +391,54 @@ int composite_upload(nr_composite &c)
     const int nb = c.gel.ks > 0 ? net.e() : 0;
     const int nt = c.gel.kb > 0 ? net.t() : 0;
     const int nf = c.gel.bmod > 0 ? c.mesh.nf() : 0;
+    // device order of the points (Morton), and every point index in it;
+    // elements keep the file's order
+    const std::vector<int> order = morton_order(n, s.pos);
+    c.dev_of.assign((size_t)n, 0);
+    for (int d = 0; d < n; d++) c.dev_of[order[d]] = d;
+    std::vector<int> dbonds(2 * (size_t)nb), dtri(3 * (size_t)nt), dfac(3 * (size_t)nf);
+    for (size_t k = 0; k < dbonds.size(); k++) dbonds[k] = c.dev_of[net.bonds[k]];
+    for (size_t k = 0; k < dtri.size(); k++) dtri[k] = c.dev_of[net.tri[k]];
+    for (size_t k = 0; k < dfac.size(); k++) dfac[k] = c.dev_of[c.mesh.facets[k]];
+    std::vector<double> dpos(2 * (size_t)n);
+    std::vector<unsigned char> dfcode((size_t)n);
+    for (int d = 0; d < n; d++) {
+        dpos[2 * (size_t)d] = s.pos[2 * (size_t)order[d]];
+        dpos[2 * (size_t)d + 1] = s.pos[2 * (size_t)order[d] + 1];
+        dfcode[d] = net.fcode[order[d]];
+    }
     std::vector<int> ent, cnt;
     long sec[3];
-    int rc = build_gather(n, net.bonds, nb, net.tri, nt, c.mesh.facets, nf, ent, cnt, sec);
+    int rc = build_gather(n, dbonds, nb, dtri, nt, dfac, nf, ent, cnt, sec);
     if (rc) return rc;
     std::vector<int2> bond((size_t)nb);
     std::vector<double2> geo((size_t)nb);
     for (int b = 0; b < nb; b++) {
-        bond[b] = make_int2(net.bonds[2 * b], net.bonds[2 * b + 1]);
+        bond[b] = make_int2(dbonds[2 * b], dbonds[2 * b + 1]);
         geo[b] = make_double2(net.rest[b], s.width * net.bwrap[2 * b]);
     }
     std::vector<int4> tri((size_t)nt);
     std::vector<double2> toff((size_t)nt);
     for (int t = 0; t < nt; t++) {
-        tri[t] = make_int4(net.tri[3 * t], net.tri[3 * t + 1], net.tri[3 * t + 2], 0);
+        tri[t] = make_int4(dtri[3 * t], dtri[3 * t + 1], dtri[3 * t + 2], 0);
         toff[t] = make_double2(s.width * net.twrap[4 * t], s.width * net.twrap[4 * t + 2]);
     }
     std::vector<int4> fac((size_t)nf);
-    for (int f = 0; f < nf; f++)
-        fac[f] = make_int4(c.mesh.facets[3 * f], c.mesh.facets[3 * f + 1], c.mesh.facets[3 * f + 2],
-                           0);
+    for (int f = 0; f < nf; f++) fac[f] = make_int4(dfac[3 * f], dfac[3 * f + 1], dfac[3 * f + 2], 0);
     std::vector<double> inv(s.inv.begin(), s.inv.begin() + 6 * (size_t)nf);
     std::vector<double> area(c.mesh.areas.begin(), c.mesh.areas.begin() + nf);
     std::vector<int> twin;
     if (nf > 0 && c.mesh.np() > 0) {  // duplicate_forces runs with the gel only (:946-951)
         twin.assign((size_t)n, -1);
         for (int p = 0; p < c.mesh.np(); p++) {
-            twin[(size_t)c.mesh.twins[2 * p]] = c.mesh.twins[2 * p + 1];
-            twin[(size_t)c.mesh.twins[2 * p + 1]] = c.mesh.twins[2 * p];
+            const int a = c.dev_of[c.mesh.twins[2 * p]], b = c.dev_of[c.mesh.twins[2 * p + 1]];
+            twin[(size_t)a] = b;
+            twin[(size_t)b] = a;
         }
     }
     const size_t nslot = (size_t)nb + 2 * (size_t)nt + 3 * (size_t)nf;
     if (nslot >= (1u << 30)) return fail(NR_E_TOPO, "composite too large for the slot index");
-    HIP_TRY(c.x.upload(s.pos));
+    HIP_TRY(c.x.upload(dpos));
     HIP_TRY(c.v.alloc(2 * (size_t)n));
     HIP_TRY(c.f.alloc(2 * (size_t)n));
     HIP_TRY(c.bond.upload(bond));
@@ -182,8 +452,43 @@ int composite_upload(nr_composite &c)
     HIP_TRY(c.ent.upload(ent));
     HIP_TRY(c.cnt.upload(cnt));
     if (!twin.empty()) HIP_TRY(c.twin.upload(twin));
-    HIP_TRY(c.fcode.upload(net.fcode));
-    HIP_TRY(c.part.alloc(4 * (size_t)kMeshMaxBlocks));
+    HIP_TRY(c.fcode.upload(dfcode));
+    // the fused element + gather kernel (NRELAX_MESH_TILES=0 at create: the
+    // element and gather kernels instead)
+    TileArrays T;
+    long tile_slots = 512;
+    const char *t_env = std::getenv("NRELAX_MESH_TILES");
+    if (!(t_env && std::strcmp(t_env, "0") == 0)) {
+        // tiles of at most kMeshTilePoints points, as few rounds of the
+        // resident workgroups (two per CU) as that allows, and the points
+        // spread evenly over the rounds (no round of a few tiles at the end)
+        int n_cu = 256;
+        hipDeviceProp_t prop;
+        if (hipGetDeviceProperties(&prop, c.device) == hipSuccess && prop.multiProcessorCount > 0)
+            n_cu = prop.multiProcessorCount;
+        const long slots = 2L * n_cu;
+        tile_slots = slots;
+        const long rounds = std::max(1L, ((long)n + slots * kMeshTilePoints - 1) /
+                                             (slots * kMeshTilePoints));
+        const int tile_points = (int)std::min<long>(kMeshTilePoints,
+                                                    std::max(32L, ((long)n + slots * rounds - 1) /
+                                                                      (slots * rounds)));
+        rc = build_tiles(n, dbonds, geo, nb, dtri, toff, nt, dfac, inv, area, nf, twin, dfcode,
+                         kMeshTileLds, tile_points, T);
+        if (rc) return rc;
+        std::vector<uint4> blob16(T.blob.size() / 16);
+        if (!T.blob.empty()) std::memcpy(blob16.data(), T.blob.data(), T.blob.size());
+        HIP_TRY(c.tiles.upload(T.tiles));
+        HIP_TRY(c.tblob.upload(blob16));
+        HIP_TRY(c.tb_ij.upload(T.b_ij));
+        HIP_TRY(c.tb_geo.upload(T.b_geo));
+        HIP_TRY(c.tt_idx.upload(T.t_idx));
+        HIP_TRY(c.tt_off.upload(T.t_off));
+        HIP_TRY(c.tf_idx.upload(T.f_idx));
+        HIP_TRY(c.tf_m.upload(T.f_m));
+        HIP_TRY(c.tf_area.upload(T.f_area));
+    }
+    HIP_TRY(c.part.alloc(4 * std::max<size_t>(kMeshMaxBlocks, T.tiles.size())));
     HIP_TRY(c.tot.alloc(4));
     HIP_TRY(c.ticket.alloc(1));
     HIP_TRY(hipMemset(c.ticket.p, 0, sizeof(unsigned)));
@@ -217,6 +522,18 @@ int composite_upload(nr_composite &c)
     m.ticket = c.ticket.p;
     m.st = c.state.p;
     m.gather_blocks = mesh_point_blocks(n);
+    m.n_tiles = (int)T.tiles.size();
+    m.tile_grid = (int)std::min<long>(m.n_tiles, tile_slots);
+    m.tile_lds = T.max_lds;
+    m.tiles = c.tiles.p;
+    m.tblob = c.tblob.p;
+    m.tb_ij = c.tb_ij.p;
+    m.tb_geo = c.tb_geo.p;
+    m.tt_idx = c.tt_idx.p;
+    m.tt_off = c.tt_off.p;
+    m.tf_idx = c.tf_idx.p;
+    m.tf_m = c.tf_m.p;
+    m.tf_area = c.tf_area.p;
     m.ks = c.gel.ks;
     m.kb = c.gel.kb;
     m.smod = c.gel.smod;
@@ -281,6 +598,21 @@ hipError_t capture_chunk(nr_composite &c)
     if (e == hipSuccess) e = hipGraphInstantiate(&c.exec, c.graph, nullptr, nullptr, 0);
     if (e == hipSuccess) c.have_graph = true;
     return e;
+}
+
+// A point array of the device (2 doubles per point, device order) into
+// `out` in the file's point order.
+int to_file_order(nr_composite &c, const double *dev, double *out)
+{
+    const size_t n = (size_t)c.md.n;
+    std::vector<double> d(2 * n);
+    HIP_TRY(hipMemcpyAsync(d.data(), dev, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, c.st));
+    HIP_TRY(hipStreamSynchronize(c.st));
+    for (size_t i = 0; i < n; i++) {
+        out[2 * i] = d[2 * (size_t)c.dev_of[i]];
+        out[2 * i + 1] = d[2 * (size_t)c.dev_of[i] + 1];
+    }
+    return NR_OK;
 }
 
 int energies(nr_composite &c, double out[3])
@@ -480,18 +812,20 @@ extern "C" int nr_composite_get_positions(nr_composite *c, double *pos)
 {
     if (!c || !pos) return nr::fail(NR_E_ARG, "null argument");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipMemcpyAsync(pos, c->x.p, sizeof(double) * 2 * (size_t)c->md.n,
-                           hipMemcpyDeviceToHost, c->st));
-    HIP_TRY(hipStreamSynchronize(c->st));
-    return NR_OK;
+    return nr::to_file_order(*c, c->x.p, pos);
 }
 
 extern "C" int nr_composite_set_positions(nr_composite *c, const double *pos)
 {
     if (!c || !pos) return nr::fail(NR_E_ARG, "null argument");
     HIP_TRY(hipSetDevice(c->device));
-    HIP_TRY(hipMemcpyAsync(c->x.p, pos, sizeof(double) * 2 * (size_t)c->md.n,
-                           hipMemcpyHostToDevice, c->st));
+    const size_t n = (size_t)c->md.n;
+    std::vector<double> d(2 * n);
+    for (size_t i = 0; i < n; i++) {
+        d[2 * (size_t)c->dev_of[i]] = pos[2 * i];
+        d[2 * (size_t)c->dev_of[i] + 1] = pos[2 * i + 1];
+    }
+    HIP_TRY(hipMemcpyAsync(c->x.p, d.data(), sizeof(double) * 2 * n, hipMemcpyHostToDevice, c->st));
     HIP_TRY(hipStreamSynchronize(c->st));
     return NR_OK;
 }
@@ -511,9 +845,10 @@ extern "C" int nr_composite_eval(nr_composite *c, double *force, double *energie
     HIP_TRY(nr::reset_state(*c, p));
     HIP_TRY(hipMemsetAsync(c->v.p, 0, sizeof(double) * 2 * (size_t)c->md.n, c->st));
     HIP_TRY((hipError_t)nr::launch_mesh_step(c->md, 0, 1, c->st));
-    if (force)
-        HIP_TRY(hipMemcpyAsync(force, c->f.p, sizeof(double) * 2 * (size_t)c->md.n,
-                               hipMemcpyDeviceToHost, c->st));
+    if (force) {
+        const int rc = nr::to_file_order(*c, c->f.p, force);
+        if (rc) return rc;
+    }
     HIP_TRY(hipStreamSynchronize(c->st));
     if (energies) {
         int rc = nr::energies(*c, energies);
