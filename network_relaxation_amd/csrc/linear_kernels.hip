@@ -26,21 +26,23 @@ namespace {
 
 constexpr int kLinThreads = 256;
 
+// Workgroup sum in a fixed order: a butterfly within each wave, then the
+// waves in order (two barriers).
 template <int B>
 __device__ inline double lin_tree_sum(double v, double *s)
 {
-    s[threadIdx.x] = v;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) s[threadIdx.x >> 6] = v;
     __syncthreads();
-    for (int o = B / 2; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) s[threadIdx.x] += s[threadIdx.x + o];
-        __syncthreads();
-    }
-    const double r = s[0];
+    double r = 0.0;
+#pragma unroll
+    for (int w = 0; w < B / 64; w++) r += s[w];
     __syncthreads();
     return r;
 }
 
-// sum of the kLinBlocks partials at p, in a fixed order (one LDS tree)
+// sum of the kLinBlocks partials at p, in a fixed order
 template <int B>
 __device__ inline double sum_partials(const double *p, double *s)
 {
