@@ -43,7 +43,9 @@ struct NetRun {
     double best, fswitch;    // FP32 stage: least max |F|^2, hand-over threshold
     double ff, fmax2;        // |F|^2 and max node |F|^2 after the last pass
     long long t_start;       // device clock when the increment began
-    long long pad;
+    int slice;               // steps this network may take in the current launch
+                             // (set by the host per launch; 0 = KArgs::slice_steps)
+    int pad;
 };
 
 struct NetOut {

@@ -1238,7 +1238,8 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     double shift = nd->le_shift;
     int p = nd->parity;
     const int max_steps = A.mode == kModeEval ? 0 : nd->max_steps;
-    const int budget = A.slice_steps > 0 ? A.slice_steps : 0x7fffffff;
+    const int slice = rs->slice > 0 ? rs->slice : A.slice_steps;
+    const int budget = slice > 0 ? slice : 0x7fffffff;
     int used = 0;
     // resumable state kept in registers; the rest of NetRun is read where needed
     int phase = rs->phase, inc_next = rs->inc_next;

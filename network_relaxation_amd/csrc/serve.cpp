@@ -124,7 +124,7 @@ struct Tables {
     DevBuf<unsigned long long> partials;
     DevBuf<unsigned long long> bar;
     std::vector<nr::BlockRole> h_roles;  // host copies, alive until the launch ends
-    std::vector<int> h_jobs, h_off;
+    std::vector<int> h_jobs, h_off, h_slice;
     int grid = 0, n_teams = 0;  // n_teams: barrier/mailbox ids in use
     bool cooperative = false;
 };
@@ -230,6 +230,20 @@ double expected_steps(const Job &j, int slice_steps)
     const double s = nr::expected_steps(j.net.n, j.plan.nets[0].n_bonds) * (double)j.incs.size();
     return std::min(s, (double)slice_steps);
 }
+
+// Cost of one FIRE step of a job in tile passes (the busiest member's tile
+// passes and the team's exchange; teams.hpp's job_cost of one step).
+double step_cost(const Job &j)
+{
+    return nr::job_cost(1.0, (int)j.plan.tiles.size(), std::max(1, j.team));
+}
+
+// Steps of each active job in the next launch: the slowest job per step gets
+// slice_steps, the others as many more as fit in the same time (at most
+// kSliceSpread times as many), so that a small job in a mix with a large one
+// advances at the pace it would alone instead of idling its workgroups until
+// the large job's slice ends.
+constexpr int kSliceSpread = 64;
 
 void release_job(nr_server *s, Job &j)
 {
@@ -410,9 +424,16 @@ int launch_slice(nr_server *s)
     s->inflight_slots = act;
     if (act.empty()) return NR_OK;
     std::vector<nr::JobSpec> specs(act.size());
+    std::vector<int> budget(act.size(), s->slice_steps);
+    double c_max = 0.0;
+    for (int k : act) c_max = std::max(c_max, step_cost(*s->jobs[s->slot_job[k]]));
     for (size_t i = 0; i < act.size(); i++) {
         const Job &j = *s->jobs[s->slot_job[act[i]]];
-        specs[i] = {j.team, (int)j.plan.tiles.size(), expected_steps(j, s->slice_steps)};
+        const double r = c_max / std::max(step_cost(j), 1e-9);
+        budget[i] = (int)std::min<double>((double)s->slice_steps * kSliceSpread,
+                                          std::max<double>(s->slice_steps,
+                                                           std::floor(s->slice_steps * r)));
+        specs[i] = {j.team, (int)j.plan.tiles.size(), expected_steps(j, budget[i])};
     }
     std::vector<nr::TeamLayout> layouts = nr::plan_launches(specs, s->resident);
     while (s->tables.size() < layouts.size()) {
@@ -463,6 +484,8 @@ int launch_slice(nr_server *s)
         t.h_jobs.resize(L.class_jobs.size());
         for (size_t i = 0; i < L.class_jobs.size(); i++) t.h_jobs[i] = act[L.class_jobs[i]];
         t.h_off = L.class_off;
+        t.h_slice.assign(act.size(), 0);
+        for (size_t i = 0; i < act.size(); i++) t.h_slice[i] = budget[i];
         t.grid = L.grid;
         t.n_teams = L.n_ids;
         t.cooperative = L.cooperative;
@@ -472,6 +495,10 @@ int launch_slice(nr_server *s)
                                hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemcpyAsync(t.class_off.p, t.h_off.data(), sizeof(int) * t.h_off.size(),
                                hipMemcpyHostToDevice, s->stream));
+        // each active job's steps of this launch, into its resumable state
+        for (size_t i = 0; i < act.size(); i++)
+            HIP_TRY(hipMemcpyAsync(&(s->run.p + act[i])->slice, &t.h_slice[i], sizeof(int),
+                                   hipMemcpyHostToDevice, s->stream));
         HIP_TRY(hipMemsetAsync(t.queues.p, 0, sizeof(int) * 32 * ncls, s->stream));
         // partial-sum records carry per-launch step tags starting at 1
         HIP_TRY(hipMemsetAsync(t.partials.p, 0, sizeof(unsigned long long) * t.partials.n,
