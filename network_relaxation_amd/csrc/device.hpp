@@ -158,13 +158,15 @@ constexpr int kAnalysisBlocks = 256;
 
 // Small-strain linear solve (linear_kernels.hip): iteration k of a chunk
 // (k from 0, chunks of an even length) of CG on K_rr w = b (CSR over n DOFs,
-// rows of clamped DOFs masked), scalars in sc (see there), kLinBlocks
+// rows of clamped DOFs masked), the direction alternating between p[k & 1]
+// (read) and p[(k & 1) ^ 1] (written), scalars in sc (see there), kLinBlocks
 // partials of p.q then r.r in part[0 .. 2 kLinBlocks); a chunk ends with
 // launch_cg_chunk_end.
 constexpr int kLinBlocks = 512;
+constexpr int kLinScalars = 9;
 int launch_cg_iteration(int n, const int *rowptr, const int *col, const double *val,
-                        const unsigned char *free_dof, double *w, double *r, double *p, double *q,
-                        double *part, double *sc, int k, int max_iter, void *stream);
+                        const unsigned char *free_dof, double *w, double *r, double *const p[2],
+                        double *q, double *part, double *sc, int k, int max_iter, void *stream);
 int launch_cg_chunk_end(double *sc, int chunk, void *stream);
 // y = P K x (clamped rows 0), partials of x.y in part[0 .. kLinBlocks).
 int launch_spmv(int n, const int *rowptr, const int *col, const double *val,

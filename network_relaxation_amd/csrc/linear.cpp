@@ -253,7 +253,7 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
         const bool done0 = !(rr0 > thr2) || max_iter == 0;
 
         DevBuf<int> d_rowptr, d_col;
-        DevBuf<double> d_val, d_w, d_r, d_p, d_q, d_part, d_sc;
+        DevBuf<double> d_val, d_w, d_r, d_p, d_p2, d_q, d_part, d_sc;
         DevBuf<unsigned char> d_free;
         HIP_TRY(d_rowptr.upload(K.rowptr));
         HIP_TRY(d_col.upload(K.col));
@@ -262,17 +262,18 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
         std::vector<double> zero(n2, 0.0);
         HIP_TRY(d_w.upload(zero));
         HIP_TRY(d_r.upload(b0));
-        HIP_TRY(d_p.upload(b0));
+        HIP_TRY(d_p.upload(zero));  // the first direction is r (beta = 0)
+        HIP_TRY(d_p2.upload(zero));
         HIP_TRY(d_q.upload(zero));
         HIP_TRY(d_part.upload(std::vector<double>(2 * nr::kLinBlocks, 0.0)));
-        HIP_TRY(d_sc.upload({rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0}));
+        HIP_TRY(d_sc.upload({rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0, 0.0}));
 
         hipStream_t st = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         hipGraph_t graph = nullptr;
         hipGraphExec_t exec = nullptr;
         int rc = NR_OK;
-        double sc[8] = {rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0};
+        double sc[nr::kLinScalars] = {rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0, 0.0};
         float ms = 0.0f;
         auto cleanup = [&]() {
             if (exec) (void)hipGraphExecDestroy(exec);
@@ -287,11 +288,12 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
         // a chunk of iterations as one graph (kernels only); replays stop doing
         // work once the device raises its done flag
         constexpr int kChunk = 32;  // even: the parity of iteration k repeats per chunk
+        double *const pbuf[2] = {d_p.p, d_p2.p};
         if (e == hipSuccess) e = hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal);
         if (e == hipSuccess) {
             for (int c = 0; c < kChunk && e == hipSuccess; c++)
                 e = (hipError_t)nr::launch_cg_iteration(n2, d_rowptr.p, d_col.p, d_val.p, d_free.p,
-                                                        d_w.p, d_r.p, d_p.p, d_q.p, d_part.p, d_sc.p,
+                                                        d_w.p, d_r.p, pbuf, d_q.p, d_part.p, d_sc.p,
                                                         c, max_iter, st);
             if (e == hipSuccess) e = (hipError_t)nr::launch_cg_chunk_end(d_sc.p, kChunk, st);
             hipError_t e2 = hipStreamEndCapture(st, &graph);
@@ -299,7 +301,7 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
         }
         if (e == hipSuccess) e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
         if (e == hipSuccess) e = hipEventRecord(e0, st);
-        while (e == hipSuccess && sc[3] == 0.0) {
+        while (e == hipSuccess && sc[3] == 0.0 && sc[8] == 0.0) {
             e = hipGraphLaunch(exec, st);
             if (e == hipSuccess)
                 e = hipMemcpyAsync(sc, d_sc.p, sizeof(sc), hipMemcpyDeviceToHost, st);
