@@ -252,22 +252,6 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
         const double thr2 = (tol * scale) * (tol * scale);
         const bool done0 = !(rr0 > thr2) || max_iter == 0;
 
-        DevBuf<int> d_rowptr, d_col;
-        DevBuf<double> d_val, d_w, d_r, d_p, d_p2, d_q, d_part, d_sc;
-        DevBuf<unsigned char> d_free;
-        HIP_TRY(d_rowptr.upload(K.rowptr));
-        HIP_TRY(d_col.upload(K.col));
-        HIP_TRY(d_val.upload(K.val));
-        HIP_TRY(d_free.upload(fr));
-        std::vector<double> zero(n2, 0.0);
-        HIP_TRY(d_w.upload(zero));
-        HIP_TRY(d_r.upload(b0));
-        HIP_TRY(d_p.upload(zero));  // the first direction is r (beta = 0)
-        HIP_TRY(d_p2.upload(zero));
-        HIP_TRY(d_q.upload(zero));
-        HIP_TRY(d_part.upload(std::vector<double>(2 * nr::kLinBlocks, 0.0)));
-        HIP_TRY(d_sc.upload({rr0, 0.0, thr2, done0 ? 1.0 : 0.0, 0.0, 0.0, 0.0, rr0, 0.0}));
-
         hipStream_t st = nullptr;
         hipEvent_t e0 = nullptr, e1 = nullptr;
         hipGraph_t graph = nullptr;
@@ -282,9 +266,40 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
             if (e1) (void)hipEventDestroy(e1);
             if (st) (void)hipStreamDestroy(st);
         };
+        // Every copy goes through this solve's own non-blocking stream: other
+        // host threads may be capturing their solves' graphs meanwhile, and a
+        // copy on the legacy stream would be refused then.
+        DevBuf<int> d_rowptr, d_col;
+        DevBuf<double> d_val, d_w, d_r, d_p, d_p2, d_q, d_part, d_sc;
+        DevBuf<unsigned char> d_free;
         hipError_t e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
         if (e == hipSuccess) e = hipEventCreate(&e0);
         if (e == hipSuccess) e = hipEventCreate(&e1);
+        auto put = [&](auto &buf, const auto &host) {
+            if (e != hipSuccess) return;
+            e = buf.alloc(host.size());
+            if (e == hipSuccess && !host.empty())
+                e = hipMemcpyAsync(buf.p, host.data(), sizeof(host[0]) * host.size(),
+                                   hipMemcpyHostToDevice, st);
+        };
+        auto zeros = [&](DevBuf<double> &buf, size_t n) {
+            if (e != hipSuccess) return;
+            e = buf.alloc(n);
+            if (e == hipSuccess) e = hipMemsetAsync(buf.p, 0, sizeof(double) * n, st);
+        };
+        const std::vector<double> sc0(sc, sc + nr::kLinScalars);
+        put(d_rowptr, K.rowptr);
+        put(d_col, K.col);
+        put(d_val, K.val);
+        put(d_free, fr);
+        put(d_r, b0);
+        put(d_sc, sc0);
+        zeros(d_w, n2);
+        zeros(d_p, n2);  // the first direction is r (beta = 0)
+        zeros(d_p2, n2);
+        zeros(d_q, n2);
+        zeros(d_part, 2 * nr::kLinBlocks);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         // a chunk of iterations as one graph (kernels only); replays stop doing
         // work once the device raises its done flag
         constexpr int kChunk = 32;  // even: the parity of iteration k repeats per chunk
@@ -312,7 +327,8 @@ int nr_linear_response(const nr_network *net, double ks, double kb, double mu, d
         if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
         std::vector<double> w(n2);
         if (e == hipSuccess)
-            e = hipMemcpy(w.data(), d_w.p, sizeof(double) * n2, hipMemcpyDeviceToHost);
+            e = hipMemcpyAsync(w.data(), d_w.p, sizeof(double) * n2, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         cleanup();
         if (e != hipSuccess) return nr::hip_fail(e, "linear solve");
 

@@ -3,6 +3,7 @@
 // (file_lists.cpp:80-137), do_relaxation_run (:792-1010) and
 // relaxation_batch (:1016-1044), with the same output files and formats.
 #include <algorithm>
+#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
@@ -13,6 +14,7 @@
 #include <regex>
 #include <sstream>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/nrelax.h"
@@ -679,6 +681,9 @@ int run_linear_protocol(const JobFiles &jf, const LinearBatchSpec &spec, bool st
 
 }  // namespace nr
 
+// Networks of a linear batch solved at the same time.
+constexpr size_t kLinearConcurrent = 4;
+
 extern "C" int nr_linear_batch(const char *batch_file, int strain_mode, int affine, int itemize)
 {
     return nr::guarded(NR_E_BATCH, [&]() -> int {
@@ -696,14 +701,32 @@ extern "C" int nr_linear_batch(const char *batch_file, int strain_mode, int affi
             std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
             return rc;
         }
+        // Networks are independent: a few at a time, each on its own host
+        // thread and device stream, so that one network's dependent CG
+        // launches overlap another's on the device; errors are reported in
+        // the file order afterwards.
+        std::vector<int> rcs(jobs.size(), NR_OK);
+        std::vector<std::string> errs(jobs.size());
+        std::atomic<size_t> next{0};
+        auto worker = [&]() {
+            for (size_t k; (k = next.fetch_add(1)) < jobs.size();)
+                rcs[k] = nr::guarded(NR_E_BATCH, [&]() -> int {
+                    return nr::run_linear_protocol(jobs[k], b, strain_mode != 0, affine != 0,
+                                                   itemize != 0, errs[k]);
+                });
+        };
+        const size_t nw = std::min<size_t>(jobs.size(), kLinearConcurrent);
+        std::vector<std::thread> pool;
+        for (size_t w = 1; w < nw; w++) pool.emplace_back(worker);
+        worker();
+        for (std::thread &t : pool) t.join();
         int first_err = NR_OK;
-        for (const nr::JobFiles &j : jobs) {
-            rc = nr::run_linear_protocol(j, b, strain_mode != 0, affine != 0, itemize != 0, err);
-            if (rc) {
-                std::fprintf(stderr, "%s: %s\n", j.dat.c_str(), err.c_str());
-                if (!first_err) first_err = rc;
+        for (size_t k = 0; k < jobs.size(); k++)
+            if (rcs[k]) {
+                std::fprintf(stderr, "%s: %s\n", jobs[k].dat.c_str(),
+                             errs[k].empty() ? "failed" : errs[k].c_str());
+                if (!first_err) first_err = rcs[k];
             }
-        }
         return first_err;
     });
 }

@@ -56,3 +56,22 @@ def test_device_linear_repeats_bit_for_bit():
     b = N.linear_response(n, 1.0, 0.0, 0.01, displacements=True)
     assert (a["iters"], a["es"], a["eb"]) == (b["iters"], b["es"], b["eb"])
     np.testing.assert_array_equal(a["u"], b["u"])
+
+
+def test_concurrent_solves_equal_sequential():
+    """Solves from several host threads at once (each on its own device
+    stream, as nrelax_linear runs a batch's networks) give the results of the
+    same solves one at a time, bit for bit."""
+    from concurrent.futures import ThreadPoolExecutor
+    nets = [N.Network.lattice(L, 0.7, s, True).triples().fiber_kappa(1e-4, 1e-2, s)
+            for L, s in ((24, 3), (32, 4), (20, 5), (28, 6))]
+
+    def solve(n):
+        return N.linear_response(n, 1.0, 0.0, 0.01, displacements=True)
+
+    seq = [solve(n) for n in nets]
+    with ThreadPoolExecutor(len(nets)) as ex:
+        conc = list(ex.map(solve, nets))
+    for a, b in zip(seq, conc):
+        assert (a["iters"], a["es"], a["eb"], a["status"]) == (b["iters"], b["es"], b["eb"], b["status"])
+        np.testing.assert_array_equal(a["u"], b["u"])
