@@ -367,6 +367,14 @@ int nr_batch_create(int count, nr_network *const *nets, const double *ks, const 
         int per_cu = 0, n_cu = 0;
         HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, b->lds, &per_cu, &n_cu));
         if (per_cu < 1) return fail(NR_E_ARG, "kernel cannot be resident with this LDS size");
+        // the whole budget, when that costs no residency: what the largest
+        // tile leaves holds tile blobs resident across steps
+        const size_t full = (size_t)(nr::kCacheBytes + lds_budget);
+        if (full > b->lds) {
+            int pc = 0, nc = 0;
+            HIP_TRY((hipError_t)nr::max_resident_blocks(b->block, b->npt, full, &pc, &nc));
+            if (pc == per_cu) b->lds = full;
+        }
         b->resident = per_cu * n_cu;
         b->team_cap = std::min({opts && opts->team > 0 ? opts->team : kDefaultTeamCap, b->resident,
                                 nr::max_team(b->block, b->npt)});
@@ -472,9 +480,7 @@ static void fill_kargs(const nr_batch *b, int mode, nr::KArgs &a)
     a.tiles = b->tiles.p;
     a.nets = b->dnet.p;
     a.out = b->dout.p;
-    a.max_blob = b->plan.max_blob;
-    a.max_local = b->plan.max_local;
-    a.max_fv = b->plan.max_fv;
+    a.tile_cap = (int)(b->lds - nr::kCacheBytes);
     a.mode = mode;
     a.err = b->err.p;
     a.timeout_ticks = 100000000ll * 60;  // 60 s at the 100 MHz constant clock

@@ -87,7 +87,8 @@ struct KArgs {
     const int *class_jobs;    // networks of each class, largest first
     const int *class_off;     // [n_classes + 1]
     int *queues;              // per class job counter, stride 32
-    int max_blob, max_local, max_fv;
+    int tile_cap;            // LDS bytes of each workgroup's tile area (blobs + work)
+    int pad_k[2];
     int mode;
     unsigned long long *partials;  // per team member kPartWords tagged words (zeroed per launch)
     unsigned long long *bar;  // per team kBarStride words: see team_arrive
@@ -102,14 +103,16 @@ struct KArgs {
 };
 
 // LDS cache, per workgroup, of the tile descriptors of the first kTileCache
-// tiles a team member relaxes in its current network and of their halo node
-// ids (a pool of kCachePool ids filled in tile order; a tile whose list no
-// longer fits is not cached), placed before the tile area: with the halo ids
-// at hand a tile pass issues its blob, own-node and halo-node loads at once.
-// Sized for 8-16 tiles of 256 nodes (~125 halo ids each) per member.
+// tiles a team member relaxes in its current network (each with where its
+// blob is held when the blobs of all of them stay resident in LDS, or else
+// where its halo ids are) and of their halo node ids (a pool of kCachePool
+// ids filled in tile order; a tile whose list no longer fits is not
+// cached), placed before the tile area: with the halo ids at hand a tile
+// pass issues its blob, own-node and halo-node loads at once.  Sized for
+// 8-16 tiles of 256 nodes (~125 halo ids each) per member.
 constexpr int kTileCache = 16;
-constexpr int kCachePool = 1792;
-constexpr int kCacheBytes = kTileCache * (int)sizeof(TileDev) + kTileCache * 4 + kCachePool * 4;
+constexpr int kCachePool = 1808;
+constexpr int kCacheBytes = kTileCache * (int)sizeof(TileDev) + kCachePool * 4;
 static_assert(kCacheBytes % 16 == 0, "tile area alignment");
 // LDS of a workgroup outside the tile area: the cache and static scratch.
 constexpr int kLdsReserve = kCacheBytes + 1024;
@@ -131,12 +134,6 @@ inline int max_team(int block, int npt) { return poll_sweeps(npt) * block / 4; }
 // [32 + 16 g] arrivals of group g (groups of kBarGroup members).
 constexpr int kBarGroup = 8;
 constexpr int kBarStride = 32 + 16 * 64;
-
-// LDS bytes of the relax kernel for a plan's maxima.
-inline size_t kernel_lds(int max_blob, int max_local, int max_fv)
-{
-    return (size_t)align16(max_blob) + 16u * max_local + 16u * max_fv;
-}
 
 // Host entry: launches the persistent relax kernel (grid workgroups of
 // `block` threads, npt own nodes per thread).  Returns a hipError_t value.

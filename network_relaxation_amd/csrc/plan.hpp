@@ -34,7 +34,10 @@ struct TileDev {
     int n_inc;       // incidence entries
     int blob16;      // blob offset in 16-byte units
     int blob_bytes;  // multiple of 16
-    int pad[3];
+    // set by the relax kernel in its LDS copy of the descriptor (0 here):
+    // where the tile's halo ids and its resident blob are held in LDS
+    int lds_halo, lds_blob;
+    int pad;
 };
 static_assert(sizeof(TileDev) == 48, "TileDev layout");
 
@@ -76,12 +79,17 @@ NR_HD inline BlobLayout blob_layout(int n_own, int n_halo, int nb, int nt, int n
     return L;
 }
 
-// LDS a tile needs beyond its blob: positions (16 B per local node) and the
-// force vectors (16 B per bond, 48 B per triple).
-inline int64_t tile_work_lds(int local, int nb, int nt)
+// LDS a tile needs beyond its blob: an FP64 pass holds positions (16 B per
+// local node) and force vectors (16 B per bond, 48 B per triple); an
+// FP32-stage pass FP64 anchors and FP32 displacements (24 B per local node)
+// and FP32 force vectors (8 B per bond, 24 B per triple).
+NR_HD inline int tile_work_bytes(int local, int nb, int nt)
 {
-    return 16ll * local + 16ll * nb + 48ll * nt;
+    const int fv = nb + 3 * nt;
+    const int w64 = 16 * local + 16 * fv, w32 = 24 * local + 8 * fv;
+    return w64 > w32 ? w64 : w32;
 }
+inline int64_t tile_work_lds(int local, int nb, int nt) { return tile_work_bytes(local, nb, nt); }
 
 struct PlanNet {
     int node_base = 0, n_nodes = 0;

@@ -5,7 +5,8 @@
 // queue is empty splits into halves that serve the next smaller class.  Per
 // FIRE step and tile:
 //   1. the tile's topology blob is copied into LDS with 16-byte loads (once
-//      per network when the member owns a single tile: it stays resident);
+//      per network when the blobs of all the member's tiles fit in LDS beside
+//      the work area: they stay resident);
 //   2. own + halo positions are advanced into LDS: the deferred FIRE mix,
 //      drift and half kick of the previous step, or, on the first pass of a
 //      relax, the affine / Lees-Edwards strain step;
@@ -397,8 +398,9 @@ struct Lds {
     unsigned char *blob;
     double2 *pos;
     double2 *fv;
-    TileDev *ctd;  // cached tile descriptors of this member
-    int *coff;     // offset of each one's halo ids in chalo, -1 = not cached
+    TileDev *ctd;  // cached tile descriptors of this member, with the offset of
+                   // each one's halo ids in chalo (lds_halo, -1: not cached) and
+                   // of its resident blob from blob (lds_blob, -1: loaded per pass)
     int *chalo;    // halo id pool
 };
 
@@ -419,6 +421,8 @@ __device__ inline TileDev tile_desc(const KArgs &A, const Lds &L0, int idx, int 
         u.n_inc = __builtin_amdgcn_readfirstlane(t.n_inc);
         u.blob16 = __builtin_amdgcn_readfirstlane(t.blob16);
         u.blob_bytes = __builtin_amdgcn_readfirstlane(t.blob_bytes);
+        u.lds_halo = __builtin_amdgcn_readfirstlane(t.lds_halo);
+        u.lds_blob = __builtin_amdgcn_readfirstlane(t.lds_blob);
     } else {
         const TileDev *t = A.tiles + idx;
         u.net = t->net;
@@ -430,6 +434,8 @@ __device__ inline TileDev tile_desc(const KArgs &A, const Lds &L0, int idx, int 
         u.n_inc = t->n_inc;
         u.blob16 = t->blob16;
         u.blob_bytes = t->blob_bytes;
+        u.lds_halo = -1;
+        u.lds_blob = -1;
     }
     return u;
 }
@@ -730,9 +736,9 @@ __device__ inline float2 advance32(const float2 u, const float2 v, const float2 
 // One FP32-stage pass over this member's tiles.  Reads the anchor from x[pa]
 // (the init pass first writes it there from x[p] and the strain step) and the
 // FP32 state from parity S.s; writes the FP32 state to parity S.s ^ 1.
-template <int B, int KIND>
+template <int B, int NPT, int KIND>
 __device__ Red4 pass32(const KArgs &A, const Team &T, const NetCtx &C, const StepScal &S,
-                       int p, int pa, const Lds &L0, bool resident)
+                       int p, int pa, const Lds &L0)
 {
     Red4 acc = {0.0, 0.0, 0.0, 0.0};
     const int tid = threadIdx.x;
@@ -743,16 +749,21 @@ __device__ Red4 pass32(const KArgs &A, const Team &T, const NetCtx &C, const Ste
     for (int ti = T.member, kt = 0; ti < C.n_tiles; ti += T.g, kt++) {
         const bool cached = kt < kTileCache;
         const TileDev td = tile_desc(A, L0, C.tile_base + ti, kt);
-        const int hoff = cached ? __builtin_amdgcn_readfirstlane(L0.coff[kt]) : -1;
+        const int hoff = td.lds_halo, roff = td.lds_blob;
+        const bool resident = roff >= 0;
         const int local = td.n_own + td.n_halo;
+        // a resident blob stays at the top of the tile area (with two nodes
+        // per thread: where a loaded one goes); a loaded one goes to its
+        // start, the anchors, displacements and force vectors after it
+        const bool top = NPT == 1 && resident;
         Lds32 L;
-        L.blob = L0.blob;
-        L.anc = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
+        L.blob = L0.blob + (top ? roff : 0);
+        L.anc = reinterpret_cast<double2 *>(L0.blob + (top ? 0 : align16(td.blob_bytes)));
         L.u = reinterpret_cast<float2 *>(L.anc + local);
         L.fv = L.u + local;
         // halo ids already in LDS (resident blob or the member's cache): the
         // blob, own-node and halo-node loads then go out together
-        const int *hid = resident ? reinterpret_cast<const int *>(L0.blob)
+        const int *hid = resident ? reinterpret_cast<const int *>(L.blob)
                          : cached && hoff >= 0 ? L0.chalo + hoff
                                                : nullptr;
         const uint4 *__restrict__ bsrc = A.blob + td.blob16;
@@ -965,7 +976,7 @@ __device__ Red4 pass32(const KArgs &A, const Team &T, const NetCtx &C, const Ste
 // the energy pass reads buffer p and returns (E_stretch, E_bend, 0, 0).
 template <int B, int NPT, int KIND>
 __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepScal &S, int p,
-                     const Lds &L0, bool resident)
+                     const Lds &L0)
 {
     Red4 acc = {0.0, 0.0, 0.0, 0.0};
     const int tid = threadIdx.x;
@@ -981,14 +992,19 @@ __device__ Red4 pass(const KArgs &A, const Team &T, const NetCtx &C, const StepS
     for (int ti = T.member, kt = 0; ti < C.n_tiles; ti += T.g, kt++) {
         const bool cached = kt < kTileCache;
         const TileDev td = tile_desc(A, L0, C.tile_base + ti, kt);
-        const int hoff = cached ? __builtin_amdgcn_readfirstlane(L0.coff[kt]) : -1;
-        // positions and force vectors follow this tile's blob
+        const int hoff = td.lds_halo, roff = td.lds_blob;
+        const bool resident = roff >= 0;
+        // a resident blob stays at the top of the tile area (with two nodes
+        // per thread: where a loaded one goes); a loaded one goes to its
+        // start, the positions and force vectors after it
+        const bool top = NPT == 1 && resident;
         Lds L = L0;
-        L.pos = reinterpret_cast<double2 *>(L0.blob + align16(td.blob_bytes));
+        L.blob = L0.blob + (top ? roff : 0);
+        L.pos = reinterpret_cast<double2 *>(L0.blob + (top ? 0 : align16(td.blob_bytes)));
         L.fv = L.pos + td.n_own + td.n_halo;
         // halo ids already in LDS (resident blob or the member's cache): the
         // blob, own-node and halo-node loads then go out together
-        const int *hid = resident ? reinterpret_cast<const int *>(L0.blob)
+        const int *hid = resident ? reinterpret_cast<const int *>(L.blob)
                          : cached && hoff >= 0 ? L0.chalo + hoff
                                                : nullptr;
         // The blob's first kBlobRound x 16 B per thread and the own nodes'
@@ -1244,20 +1260,48 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
     // resumable state kept in registers; the rest of NetRun is read where needed
     int phase = rs->phase, inc_next = rs->inc_next;
 
-    // a member owning exactly one tile keeps that tile's blob in LDS
-    const bool resident = T.member < C.n_tiles && C.n_tiles <= T.member + T.g;
-    if (resident) stage_blob<B>(A, A.tiles[C.tile_base + T.member], L.blob);
-    // descriptors and halo ids of the member's first tiles (halo ids lead the
-    // blob), the ids packed into the pool while they fit
-    for (int k = 0, ti = T.member, used = 0; k < kTileCache && ti < C.n_tiles; k++, ti += T.g) {
+    // descriptors of the member's first tiles.  When the blobs of all the
+    // member's tiles (at most kTileCache) fit at the top of the tile area
+    // with room below them for the largest work area (positions and force
+    // vectors) of those tiles, they stay resident there, back to back.  With
+    // two nodes per thread (fewer registers to spare) only a member's single
+    // tile stays, where a loaded blob goes.
+    const int n_mine = T.member < C.n_tiles ? (C.n_tiles - T.member + T.g - 1) / T.g : 0;
+    int res_bytes = 0, res_work = 0;
+    for (int k = 0, ti = T.member; k < kTileCache && ti < C.n_tiles; k++, ti += T.g) {
         const TileDev *tg = A.tiles + C.tile_base + ti;
-        constexpr int kWords = (int)(sizeof(TileDev) / sizeof(int));
+        constexpr int kWords = (int)(offsetof(TileDev, lds_halo) / sizeof(int));
         if (threadIdx.x < kWords)
             reinterpret_cast<int *>(L.ctd + k)[threadIdx.x] =
                 reinterpret_cast<const int *>(tg)[threadIdx.x];
+        res_bytes += align16(tg->blob_bytes);
+        res_work = max(res_work, tile_work_bytes(tg->n_own + tg->n_halo, tg->nb, tg->nt));
+    }
+    const bool resident = n_mine >= 1 && n_mine <= kTileCache &&
+                          (NPT == 1 ? res_bytes + res_work <= A.tile_cap : n_mine == 1);
+    // the resident blobs, or else the halo ids of the member's first tiles
+    // (halo ids lead the blob), packed into the pool while they fit
+    for (int k = 0, ti = T.member, used = 0, o = A.tile_cap; k < kTileCache && ti < C.n_tiles;
+         k++, ti += T.g) {
+        const TileDev *tg = A.tiles + C.tile_base + ti;
+        if (resident) {
+            o = NPT == 1 ? o - align16(tg->blob_bytes) : 0;
+            if (threadIdx.x == 0) {
+                L.ctd[k].lds_halo = -1;
+                L.ctd[k].lds_blob = o;
+            }
+            const uint4 *__restrict__ src = A.blob + tg->blob16;
+            uint4 *dst = reinterpret_cast<uint4 *>(L.blob + o);
+            const int n16 = tg->blob_bytes >> 4;
+            for (int i = threadIdx.x; i < n16; i += B) dst[i] = src[i];
+            continue;
+        }
         const int n_halo = tg->n_halo;
         const bool fits = used + n_halo <= kCachePool;
-        if (threadIdx.x == 0) L.coff[k] = fits ? used : -1;
+        if (threadIdx.x == 0) {
+            L.ctd[k].lds_halo = fits ? used : -1;
+            L.ctd[k].lds_blob = -1;
+        }
         if (fits) {
             const int *src = reinterpret_cast<const int *>(A.blob + tg->blob16);
             for (int i = threadIdx.x; i < n_halo; i += B) L.chalo[used + i] = src[i];
@@ -1334,7 +1378,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
             double best, fswitch;
             int since_best;
             if (fresh) {
-                part = pass32<B, kPass32Init>(A, T, C, S, p, pa, L, resident);
+                part = pass32<B, NPT, kPass32Init>(A, T, C, S, p, pa, L);
                 if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
                 S.s = 1;
                 S.pend_x = 0.0;
@@ -1355,7 +1399,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 step++;
                 used++;
                 fire.load(S);
-                part = pass32<B, kPass32Step>(A, T, C, S, p, pa, L, resident);
+                part = pass32<B, NPT, kPass32Step>(A, T, C, S, p, pa, L);
                 if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) {
                     status = NR_ABORTED;
                     break;
@@ -1381,7 +1425,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 // the velocity carried over by the fuse
                 steps32 = step;
                 p = pa;
-                part = pass<B, NPT, kPassFuse>(A, T, C, S, p, L, resident);
+                part = pass<B, NPT, kPassFuse>(A, T, C, S, p, L);
                 if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
                 p ^= 1;
                 ff = r.c;
@@ -1393,7 +1437,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 }
             }
         } else if (fresh) {
-            part = pass<B, NPT, kPassInit>(A, T, C, S, p, L, resident);
+            part = pass<B, NPT, kPassInit>(A, T, C, S, p, L);
             if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) status = NR_ABORTED;
             S.pend_x = 0.0;
             S.pend_y = 0.0;
@@ -1455,7 +1499,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 for (; ls < kCgLineSearch; ls++) {
                     S.cg_a = a;
                     S.cg_first = ls == 0;
-                    part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L, resident);
+                    part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L);
                     if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, tr, s_red, s_flag)) {
                         status = NR_ABORTED;
                         break;
@@ -1473,7 +1517,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 if (ls == kCgLineSearch) {  // evaluate the last step length
                     S.cg_a = a;
                     S.cg_first = 0;
-                    part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L, resident);
+                    part = pass<B, NPT, kPassCgTrial>(A, T, C, S, p, L);
                     if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, tr, s_red, s_flag)) {
                         status = NR_ABORTED;
                         break;
@@ -1513,7 +1557,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
                 step++;
                 used++;
                 fire.load(S);
-                part = pass<B, NPT, kPassStep>(A, T, C, S, p, L, resident);
+                part = pass<B, NPT, kPassStep>(A, T, C, S, p, L);
                 if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, r, s_red, s_flag)) {
                     status = NR_ABORTED;
                     break;
@@ -1544,7 +1588,7 @@ __device__ void relax_network(const KArgs &A, Team &T, int net_id, const Lds &L,
         // energies at the final positions (:723-757)
         Red4 e = {0.0, 0.0, 0.0, 0.0};
         if (status != NR_ABORTED) {
-            part = pass<B, NPT, kPassEnergy>(A, T, C, S, p, L, resident);
+            part = pass<B, NPT, kPassEnergy>(A, T, C, S, p, L);
             if (!reduce4<B, poll_sweeps(NPT)>(A, T, part, e, s_red, s_flag)) status = NR_ABORTED;
         }
         if (lead) {
@@ -1612,8 +1656,7 @@ __device__ __forceinline__ void kernel_body(const KArgs &A)
     __shared__ int s_job;
     Lds L;
     L.ctd = reinterpret_cast<TileDev *>(smem);
-    L.coff = reinterpret_cast<int *>(smem + kTileCache * sizeof(TileDev));
-    L.chalo = L.coff + kTileCache;
+    L.chalo = reinterpret_cast<int *>(smem + kTileCache * sizeof(TileDev));
     L.blob = smem + kCacheBytes;
     L.pos = nullptr;  // placed per tile, after its blob
     L.fv = nullptr;

@@ -467,6 +467,7 @@ int launch_slice(nr_server *s)
     a.tiles = s->tiles.p;
     a.nets = s->dnet.p;
     a.out = s->dout.p;
+    a.tile_cap = (int)(s->lds - nr::kCacheBytes);
     a.mode = nr::kModeRelax;
     a.err = s->err.p;
     a.timeout_ticks = 100000000ll * 60;
