@@ -42,15 +42,15 @@ HBM_PEAK_GBS = 8000.0
 # L 16..192, |F|max < 1e-9, tuned shapes; a relax planned from the previous
 # one's step counts), from the relax kernel's own request counters
 # (rocprofv3 --pmc): reads 32 RDREQ_32B + 64 RDREQ_64B + 128 RDREQ_128B =
-# 64 x 182,099,468 + 128 x 193,932,631,984, writes 64 WRREQ_64B + 32 (WRREQ -
-# WRREQ_64B) = 64 x 102,990,769,603 + 32 x 489,644,433 (the same sums give
+# 64 x 182,394,455 + 128 x 193,891,664,407, writes 64 WRREQ_64B + 32 (WRREQ -
+# WRREQ_64B) = 64 x 102,991,165,248 + 32 x 489,643,846 (the same sums give
 # exactly 1 GiB for a 1 GiB device copy).
-PROFILED_TRAFFIC_BYTES = (64 * 182099468 + 128 * 193932631984 +
-                          64 * 102990769603 + 32 * (103480414036 - 102990769603))
+PROFILED_TRAFFIC_BYTES = (64 * 182394455 + 128 * 193891664407 +
+                          64 * 102991165248 + 32 * (103480809094 - 102991165248))
 # The same counters on one L=512 network's FIRE ramp (p 0.6, kappa 1e-4):
-# counted HBM bytes per FIRE step (5.6666e12 read + 1.6760e12 written over
-# 131,823 steps).
-RAMP512_TRAFFIC_PER_STEP = (64 * 518663686 + 128 * 44010674284 +
+# counted HBM bytes per FIRE step (3.4910e12 read + 1.6760e12 written over
+# 131,823 steps; the tile blobs stay resident in LDS).
+RAMP512_TRAFFIC_PER_STEP = (64 * 565665115 + 128 * 26990934960 +
                             64 * 25917849604 + 32 * (26457814164 - 25917849604)) / 131823
 FIRE = dict(alpha0=0.1, f_alpha=0.99, dt_max=0.5, f_inc=1.1, f_dec=0.5, nmin=5, mass=1.0)
 
