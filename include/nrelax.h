@@ -264,6 +264,22 @@ typedef struct {
 int nr_linear_response(const nr_network *net, double ks, double kb, double mu, double shear,
                        double compression, int affine, double tol, int max_iter, double *u,
                        nr_linear_report *rep);
+/* Several load steps, of one network or several (the reference solves them
+ * one after another in do_relaxation_run :676-881 and run_relaxation_batch
+ * :1040-1090), solved at once: every CG launch serves all of them, each with
+ * its own scalars, stopping rule and workgroups, and the systems of one
+ * network and stiffness share its matrix.  Each job's results equal, bit for
+ * bit, those of nr_linear_response with the same arguments (alone or in any
+ * batch); device_ms is the whole batch's device time. */
+typedef struct {
+    const nr_network *net;
+    double ks, kb, mu, shear, compression;
+    int affine;
+    double tol;
+    int max_iter;
+    double *u;          /* NULL or 2N doubles: bc + w */
+} nr_linear_job;
+int nr_linear_response_batch(int count, const nr_linear_job *jobs, nr_linear_report *reps);
 /* The small-strain batch of the reference program `sp_relax`
  * (run_relaxation_batch, sparse_relax.cpp:1040-1090, and do_relaxation_run
  * :676-881): the 4-line batch file (base [tag]; ks kb mu; compression min max

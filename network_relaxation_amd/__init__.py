@@ -89,6 +89,12 @@ class LinearReport(C.Structure):
                 ("iterations", C.c_int), ("status", C.c_int), ("device_ms", C.c_double)]
 
 
+class LinearJob(C.Structure):
+    _fields_ = [("net", C.c_void_p), ("ks", C.c_double), ("kb", C.c_double), ("mu", C.c_double),
+                ("shear", C.c_double), ("compression", C.c_double), ("affine", C.c_int),
+                ("tol", C.c_double), ("max_iter", C.c_int), ("u", C.POINTER(C.c_double))]
+
+
 class MeshInfo(C.Structure):
     _fields_ = [("n_points", C.c_int), ("n_bonds", C.c_int), ("n_facets", C.c_int),
                 ("n_twins", C.c_int), ("offset", C.c_double), ("minx", C.c_double),
@@ -151,6 +157,7 @@ def lib():
         L.nr_linear_response.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_double,
                                          C.c_double, C.c_int, C.c_double, C.c_int, dp,
                                          P(LinearReport)]
+        L.nr_linear_response_batch.argtypes = [C.c_int, P(LinearJob), P(LinearReport)]
         L.nr_batch_last_ms.argtypes = [vp]
         L.nr_batch_last_ms.restype = C.c_double
         L.nr_batch_counts.argtypes = [vp, C.c_int, P(C.c_int64), P(C.c_int64), P(C.c_int64)]
@@ -507,6 +514,35 @@ def linear_response(net, ks, kb, shear, compression=0.0, mu=0.0, affine=True, to
                iters=rep.iterations, status=rep.status, device_ms=rep.device_ms)
     if displacements:
         out["u"] = u
+    return out
+
+
+def linear_response_batch(jobs, displacements=False):
+    """Several small-strain solves at once on the current device
+    (nr_linear_response_batch).  jobs: dicts with net, ks, kb, shear and
+    optionally compression, mu, affine, tol, max_iter (linear_response's
+    defaults).  Returns one result per job, as linear_response does; each
+    equals linear_response's for the same arguments bit for bit."""
+    n = len(jobs)
+    arr = (LinearJob * max(n, 1))()
+    us = []
+    for k, j in enumerate(jobs):
+        u = np.zeros(2 * j["net"].info().n_nodes) if displacements else None
+        us.append(u)
+        arr[k] = LinearJob(j["net"].h, j["ks"], j["kb"], j.get("mu", 0.0), j["shear"],
+                           j.get("compression", 0.0), int(j.get("affine", True)),
+                           j.get("tol", 1e-12), j.get("max_iter", 1000000),
+                           _dp(u) if displacements else None)
+    reps = (LinearReport * max(n, 1))()
+    _check(lib().nr_linear_response_batch(n, arr, reps))
+    out = []
+    for k in range(n):
+        r = reps[k]
+        d = dict(es=r.e_stretch, eb=r.e_bend, residual=r.residual, iters=r.iterations,
+                 status=r.status, device_ms=r.device_ms)
+        if displacements:
+            d["u"] = us[k]
+        out.append(d)
     return out
 
 

@@ -153,22 +153,34 @@ int launch_zero(void *p, size_t bytes, void *stream);
 // the network's device node indices, edge order.
 constexpr int kAnalysisBlocks = 256;
 
-// Small-strain linear solve (linear_kernels.hip): iteration k of a chunk
-// (k from 0, chunks of an even length) of CG on K_rr w = b (CSR over n DOFs,
-// rows of clamped DOFs masked), the direction alternating between p[k & 1]
-// (read) and p[(k & 1) ^ 1] (written), scalars in sc (see there), kLinBlocks
-// partials of p.q then r.r in part[0 .. 2 kLinBlocks); a chunk ends with
+// Small-strain linear solve (linear_kernels.hip): a batch of independent
+// systems (segments).  Segment i solves over rows [row0, row0 + n) of the
+// batch's vectors with the CSR matrix whose n + 1 row offsets start at
+// rowptr[rp0] (offsets relative to nz0 in col and val, columns local to the
+// segment) and whose free-DOF mask starts at free_dof[fr0] (rows of clamped
+// DOFs masked), served by workgroups [blk0, blk0 + nblk) of each launch
+// (nblk = lin_segment_blocks(n)); seg_of maps a workgroup to its segment.
+// Segments of one network and stiffness share its matrix.  Iteration k of a
+// chunk (k from 0, chunks of an even length) of CG on K_rr w = b, the
+// direction alternating between p[k & 1] (read) and p[(k & 1) ^ 1]
+// (written), kLinScalars scalars per segment (see linear_kernels.hip),
+// partials of p.q then r.r in part[0 .. 2 blocks); a chunk ends with
 // launch_cg_chunk_end.
-constexpr int kLinBlocks = 512;
+struct LinSeg {
+    int row0, n, blk0, nblk;
+    int rp0, fr0;
+    int max_iter;  // iterations the segment may take
+    int pad;
+    long long nz0;
+};
+constexpr int kLinBlocks = 512;  // most workgroups of one segment
 constexpr int kLinScalars = 9;
-int launch_cg_iteration(int n, const int *rowptr, const int *col, const double *val,
-                        const unsigned char *free_dof, double *w, double *r, double *const p[2],
-                        double *q, double *part, double *sc, int k, int max_iter, void *stream);
-int launch_cg_chunk_end(double *sc, int chunk, void *stream);
-// y = P K x (clamped rows 0), partials of x.y in part[0 .. kLinBlocks).
-int launch_spmv(int n, const int *rowptr, const int *col, const double *val,
-                const unsigned char *free_dof, const double *x, double *y, double *part,
-                void *stream);
+int lin_segment_blocks(int n);
+int launch_cg_iteration(const LinSeg *segs, const int *seg_of, int blocks, const int *rowptr,
+                        const int *col, const double *val, const unsigned char *free_dof,
+                        double *w, double *r, double *const p[2], double *q, double *part,
+                        double *sc, int k, void *stream);
+int launch_cg_chunk_end(double *sc, int count, int chunk, void *stream);
 // strain may be null (the bond strains are then not stored).
 int launch_analysis(const double2 *x, const double2 *x0, int n, double miny, double shear,
                     double axial, const int4 *bonds, int ne, double W, double H, double shift,

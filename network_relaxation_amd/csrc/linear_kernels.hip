@@ -1,8 +1,10 @@
 // gfx950 kernels of the small-strain linear solve (the reference's
 // sparse_relax.cpp, whose SuiteSparse QR is replaced by conjugate gradients on
 // the device): K_rr w = b over the free DOFs, K the assembled stiffness in
-// CSR (FP64), all vectors and scalars resident.  One CG iteration k is two
-// launches:
+// CSR (FP64), all vectors and scalars resident.  A launch serves a batch of
+// independent systems (segments: networks, load steps), each with its own
+// workgroups, partial sums, scalars and stopping, so that a batch pays the
+// launch latency of an iteration once.  One CG iteration k is two launches:
 //   cg_dir_spmv  r.r of the current r (from the last update's partials), the
 //                stopping rule, beta = r.r / r.r_old, then for its rows
 //                p = r + beta p_old (written to the other p buffer) and
@@ -11,19 +13,22 @@
 //   cg_update    alpha = r.r / p.q; w += alpha p; r -= alpha q; partials of r.r
 // (the direction update of the textbook iteration, fused into the SpMV, so
 // that an iteration waits for two launches instead of three; the arithmetic
-// is the textbook order's, value for value).  Scalars are re-reduced from
-// kLinBlocks per-workgroup partials in a fixed order by every workgroup that
-// needs them (no atomics; the result does not depend on scheduling).
-// Iterations are captured in chunks of an even number k = 0, 1, ...: r.r and
-// the done flag are kept per iteration parity (k & 1), and p alternates
-// between two buffers by the same parity, so no kernel writes what another
-// workgroup of the same launch reads.  Once a done flag is up every kernel
-// returns at once (cg_dir_spmv carrying it forward), so replays never
-// overshoot.  Scalars sc: [0, 1] r.r by parity, [2] (tol x scale)^2, [3, 4]
-// done before iteration k by parity, [5] iterations before this chunk,
-// [6] iterations when done, [7] the last r.r, [8] set when p.q <= 0 (no
-// descent left).
+// is the textbook order's, value for value).  A segment's scalars are
+// re-reduced from its workgroups' partials in a fixed order by every one of
+// its workgroups that needs them (no atomics; the result depends neither on
+// scheduling nor on the other segments of the batch).  Iterations are
+// captured in chunks of an even number k = 0, 1, ...: r.r and the done flag
+// are kept per iteration parity (k & 1), and p alternates between two
+// buffers by the same parity, so no kernel writes what another workgroup of
+// the same launch reads.  Once a segment's done flag is up its workgroups
+// return at once (cg_dir_spmv carrying the flag forward), so replays never
+// overshoot.  Scalars of a segment (kLinScalars): [0, 1] r.r by parity,
+// [2] (tol x scale)^2, [3, 4] done before iteration k by parity, [5]
+// iterations before this chunk, [6] iterations when done, [7] the last r.r,
+// [8] set when p.q <= 0 (no descent left).
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 
 #include "device.hpp"
 
@@ -48,53 +53,35 @@ __device__ inline double lin_tree_sum(double v, double *s)
     return r;
 }
 
-// sum of the kLinBlocks partials at p, in a fixed order
+// sum of the count partials at p, in a fixed order
 template <int B>
-__device__ inline double sum_partials(const double *p, double *s)
+__device__ inline double sum_partials(const double *p, int count, double *s)
 {
     double v = 0.0;
-    for (int i = threadIdx.x; i < kLinBlocks; i += B) v += p[i];
+    for (int i = threadIdx.x; i < count; i += B) v += p[i];
     return lin_tree_sum<B>(v, s);
 }
 
 template <int B>
-__global__ __launch_bounds__(B) void cg_spmv(int n, const int *__restrict__ rowptr,
-                                             const int *__restrict__ col,
-                                             const double *__restrict__ val,
-                                             const unsigned char *__restrict__ free_dof,
-                                             const double *__restrict__ p, double *__restrict__ q,
-                                             double *part, const double *sc, int par)
-{
-    __shared__ double s[B];
-    if (sc && sc[3 + par] != 0.0) return;
-    double acc = 0.0;
-    for (int r = blockIdx.x * B + threadIdx.x; r < n; r += gridDim.x * B) {
-        double v = 0.0;
-        if (free_dof[r]) {
-            const int lo = rowptr[r], hi = rowptr[r + 1];
-            for (int k = lo; k < hi; k++) v = fma(val[k], p[col[k]], v);
-        }
-        q[r] = v;
-        acc = fma(p[r], v, acc);
-    }
-    acc = lin_tree_sum<B>(acc, s);
-    if (threadIdx.x == 0) part[blockIdx.x] = acc;
-}
-
-template <int B>
-__global__ __launch_bounds__(B) void cg_dir_spmv(int n, const int *__restrict__ rowptr,
+__global__ __launch_bounds__(B) void cg_dir_spmv(const LinSeg *__restrict__ segs,
+                                                 const int *__restrict__ seg_of,
+                                                 const int *__restrict__ rowptr,
                                                  const int *__restrict__ col,
                                                  const double *__restrict__ val,
                                                  const unsigned char *__restrict__ free_dof,
                                                  const double *__restrict__ r,
                                                  const double *__restrict__ p_old,
                                                  double *__restrict__ p_new, double *__restrict__ q,
-                                                 double *part_pq, const double *part_rr, double *sc,
-                                                 int k, int max_iter)
+                                                 double *part_pq, const double *part_rr,
+                                                 double *sc_all, int k)
 {
     __shared__ double s[B];
+    const int sid = seg_of[blockIdx.x];
+    const LinSeg S = segs[sid];
+    double *sc = sc_all + (size_t)kLinScalars * sid;
+    const int lb = blockIdx.x - S.blk0;
     const int par = k & 1;
-    const bool lead = blockIdx.x == 0 && threadIdx.x == 0;
+    const bool lead = lb == 0 && threadIdx.x == 0;
     if (sc[3 + par] != 0.0 || sc[8] != 0.0) {
         if (lead) sc[3 + (par ^ 1)] = 1.0;
         return;
@@ -102,8 +89,8 @@ __global__ __launch_bounds__(B) void cg_dir_spmv(int n, const int *__restrict__ 
     const double it = sc[5] + (double)k;  // iterations done
     const bool first = it == 0.0;
     // r.r of the current r: the host's for the first iteration
-    const double rr1 = first ? sc[par] : sum_partials<B>(part_rr, s);
-    if (!(rr1 > sc[2]) || it >= (double)max_iter) {
+    const double rr1 = first ? sc[par] : sum_partials<B>(part_rr + S.blk0, S.nblk, s);
+    if (!(rr1 > sc[2]) || it >= (double)S.max_iter) {
         if (lead) {
             sc[3 + (par ^ 1)] = 1.0;
             sc[6] = it;
@@ -113,19 +100,25 @@ __global__ __launch_bounds__(B) void cg_dir_spmv(int n, const int *__restrict__ 
     }
     const double rr0 = sc[par];
     const double beta = !first && rr0 > 0.0 ? rr1 / rr0 : 0.0;
+    const int *__restrict__ rp = rowptr + S.rp0;
+    const int *__restrict__ cl = col + S.nz0;
+    const double *__restrict__ vl = val + S.nz0;
+    const unsigned char *__restrict__ fr = free_dof + S.fr0;
+    const double *__restrict__ rs = r + S.row0;
+    const double *__restrict__ po = p_old + S.row0;
     double acc = 0.0;
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
-        const double pi = fma(beta, p_old[i], r[i]);
-        p_new[i] = pi;
+    for (int i = lb * B + threadIdx.x; i < S.n; i += S.nblk * B) {
+        const double pi = fma(beta, po[i], rs[i]);
+        p_new[S.row0 + i] = pi;
         double v = 0.0;
-        if (free_dof[i]) {
-            const int lo = rowptr[i], hi = rowptr[i + 1];
+        if (fr[i]) {
+            const int lo = rp[i], hi = rp[i + 1];
             for (int e = lo; e < hi; e++) {
-                const int j = col[e];
-                v = fma(val[e], fma(beta, p_old[j], r[j]), v);
+                const int j = cl[e];
+                v = fma(vl[e], fma(beta, po[j], rs[j]), v);
             }
         }
-        q[i] = v;
+        q[S.row0 + i] = v;
         acc = fma(pi, v, acc);
     }
     acc = lin_tree_sum<B>(acc, s);
@@ -137,17 +130,23 @@ __global__ __launch_bounds__(B) void cg_dir_spmv(int n, const int *__restrict__ 
 }
 
 template <int B>
-__global__ __launch_bounds__(B) void cg_update(int n, const double *__restrict__ p,
+__global__ __launch_bounds__(B) void cg_update(const LinSeg *__restrict__ segs,
+                                               const int *__restrict__ seg_of,
+                                               const double *__restrict__ p,
                                                const double *__restrict__ q, double *w, double *r,
-                                               const double *part_pq, double *part_rr, double *sc,
-                                               int k)
+                                               const double *part_pq, double *part_rr,
+                                               double *sc_all, int k)
 {
     __shared__ double s[B];
+    const int sid = seg_of[blockIdx.x];
+    const LinSeg S = segs[sid];
+    double *sc = sc_all + (size_t)kLinScalars * sid;
+    const int lb = blockIdx.x - S.blk0;
     const int par = k & 1;
     if (sc[3 + (par ^ 1)] != 0.0 || sc[8] != 0.0) return;
-    const double pq = sum_partials<B>(part_pq, s);
+    const double pq = sum_partials<B>(part_pq + S.blk0, S.nblk, s);
     if (!(pq > 0.0)) {  // no descent left: the iteration does not move w
-        if (blockIdx.x == 0 && threadIdx.x == 0) {
+        if (lb == 0 && threadIdx.x == 0) {
             sc[8] = 1.0;
             sc[6] = sc[5] + (double)k;
         }
@@ -155,7 +154,7 @@ __global__ __launch_bounds__(B) void cg_update(int n, const double *__restrict__
     }
     const double alpha = sc[par ^ 1] / pq;
     double acc = 0.0;
-    for (int i = blockIdx.x * B + threadIdx.x; i < n; i += gridDim.x * B) {
+    for (int i = S.row0 + lb * B + threadIdx.x; i < S.row0 + S.n; i += S.nblk * B) {
         w[i] = fma(alpha, p[i], w[i]);
         const double ri = fma(-alpha, q[i], r[i]);
         r[i] = ri;
@@ -165,43 +164,44 @@ __global__ __launch_bounds__(B) void cg_update(int n, const double *__restrict__
     if (threadIdx.x == 0) part_rr[blockIdx.x] = acc;
 }
 
-// iterations before the next chunk (after the chunk's last update)
-__global__ void cg_chunk_end(double *sc, int chunk)
+// iterations before the next chunk (after the chunk's last update), per segment
+__global__ void cg_chunk_end(double *sc, int count, int chunk)
 {
-    if (threadIdx.x == 0) sc[5] += (double)chunk;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) sc[(size_t)kLinScalars * i + 5] += (double)chunk;
 }
 
 }  // namespace
 
-int launch_cg_iteration(int n, const int *rowptr, const int *col, const double *val,
-                        const unsigned char *free_dof, double *w, double *r, double *const p[2],
-                        double *q, double *part, double *sc, int k, int max_iter, void *stream)
+int lin_segment_blocks(int n)
 {
+    return std::max(1, std::min(kLinBlocks, (n + kLinThreads - 1) / kLinThreads));
+}
+
+int launch_cg_iteration(const LinSeg *segs, const int *seg_of, int blocks, const int *rowptr,
+                        const int *col, const double *val, const unsigned char *free_dof,
+                        double *w, double *r, double *const p[2], double *q, double *part,
+                        double *sc, int k, void *stream)
+{
+    if (blocks <= 0) return 0;
     hipStream_t st = (hipStream_t)stream;
-    double *part_pq = part, *part_rr = part + kLinBlocks;
+    double *part_pq = part, *part_rr = part + blocks;
     const int par = k & 1;
-    hipLaunchKernelGGL(cg_dir_spmv<kLinThreads>, dim3(kLinBlocks), dim3(kLinThreads), 0, st, n,
-                       rowptr, col, val, free_dof, (const double *)r, (const double *)p[par],
-                       p[par ^ 1], q, part_pq, (const double *)part_rr, sc, k, max_iter);
-    hipLaunchKernelGGL(cg_update<kLinThreads>, dim3(kLinBlocks), dim3(kLinThreads), 0, st, n,
-                       (const double *)p[par ^ 1], (const double *)q, w, r,
+    hipLaunchKernelGGL(cg_dir_spmv<kLinThreads>, dim3(blocks), dim3(kLinThreads), 0, st, segs,
+                       seg_of, rowptr, col, val, free_dof, (const double *)r,
+                       (const double *)p[par], p[par ^ 1], q, part_pq, (const double *)part_rr,
+                       sc, k);
+    hipLaunchKernelGGL(cg_update<kLinThreads>, dim3(blocks), dim3(kLinThreads), 0, st, segs,
+                       seg_of, (const double *)p[par ^ 1], (const double *)q, w, r,
                        (const double *)part_pq, part_rr, sc, k);
     return (int)hipGetLastError();
 }
 
-int launch_cg_chunk_end(double *sc, int chunk, void *stream)
+int launch_cg_chunk_end(double *sc, int count, int chunk, void *stream)
 {
-    hipLaunchKernelGGL(cg_chunk_end, dim3(1), dim3(64), 0, (hipStream_t)stream, sc, chunk);
-    return (int)hipGetLastError();
-}
-
-int launch_spmv(int n, const int *rowptr, const int *col, const double *val,
-                const unsigned char *free_dof, const double *x, double *y, double *part,
-                void *stream)
-{
-    hipLaunchKernelGGL(cg_spmv<kLinThreads>, dim3(kLinBlocks), dim3(kLinThreads), 0,
-                       (hipStream_t)stream, n, rowptr, col, val, free_dof, x, y, part,
-                       (const double *)nullptr, 0);
+    if (count <= 0) return 0;
+    hipLaunchKernelGGL(cg_chunk_end, dim3((count + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       sc, count, chunk);
     return (int)hipGetLastError();
 }
 

@@ -3,18 +3,17 @@
 // (file_lists.cpp:80-137), do_relaxation_run (:792-1010) and
 // relaxation_batch (:1016-1044), with the same output files and formats.
 #include <algorithm>
-#include <atomic>
 #include <cfloat>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <ctime>
+#include <memory>
 #include <filesystem>
 #include <fstream>
 #include <regex>
 #include <sstream>
 #include <string>
-#include <thread>
 #include <vector>
 
 #include "../../include/nrelax.h"
@@ -587,102 +586,132 @@ int parse_linear_batch(const std::string &path, LinearBatchSpec &b, std::string 
     return NR_OK;
 }
 
-// do_relaxation_run of sparse_relax.cpp (:676-881): for every compression,
-// then shear, of the load grid, one small-strain solve from the input
-// geometry; per load step a residual line in the log, an energy line and a
-// displacement file.  In strain mode the compression is scaled by the y
-// extent and the shear by the x extent (:731-739), then both are divided by
-// the y extent to form the affine field (:785), as the reference does.
-int run_linear_protocol(const JobFiles &jf, const LinearBatchSpec &spec, bool strain_mode,
-                        bool affine, bool itemize, std::string &err)
-{
-    const std::time_t start = std::time(nullptr);
+// do_relaxation_run of sparse_relax.cpp (:676-881) for one network: for
+// every compression, then shear, of the load grid, one small-strain solve
+// from the input geometry; per load step a residual line in the log, an
+// energy line and a displacement file.  In strain mode the compression is
+// scaled by the y extent and the shear by the x extent (:731-739), then both
+// are divided by the y extent to form the affine field (:785), as the
+// reference does.  plan() reads the network and lists its solves (jobs), the
+// caller solves them (with other networks' in one device batch), write()
+// writes the outputs.
+struct LinearRun {
+    const JobFiles *jf = nullptr;
+    std::time_t start = 0;
     nr_network *net = nullptr;
-    int rc = nr_network_read(jf.dat.c_str(), &net);
-    if (rc) {
-        err = nr_last_error();
-        return rc;
-    }
-    struct Guard {
-        nr_network *n = nullptr;
-        ~Guard() { nr_network_free(n); }
-    } g;
-    g.n = net;
-    if (spec.kb > 0) nr_network_build_triples(net);
-    nr_network_info info;
-    nr_network_get_info(net, &info);
-    std::vector<double> pos(2 * (size_t)info.n_nodes);
-    nr_network_get_arrays(net, pos.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
-                          nullptr);
-    double minx = DBL_MAX, maxx = -DBL_MAX;
-    for (int i = 0; i < info.n_nodes; i++) {
-        minx = std::min(minx, pos[2 * i]);
-        maxx = std::max(maxx, pos[2 * i]);
-    }
-    const double yext = info.maxy - info.miny, xext = maxx - minx;
-    // equal ends: one step (run_relaxation_batch :1081-1082), before scaling
-    double cmin = spec.cmin, cmax = spec.cmax, cinc = spec.cinc;
-    double smin = spec.smin, smax = spec.smax, sinc = spec.sinc;
-    if (smax - smin < kFltTol) sinc = 1e12;
-    if (cmax - cmin < kFltTol) cinc = 1e12;
-    if (strain_mode) {
-        cmin *= yext;
-        cmax *= yext;
-        cinc *= yext;
-        smin *= xext;
-        smax *= xext;
-        sinc *= xext;
-    }
-    if (!(cinc > 0) || !(sinc > 0)) {
-        err = "a load range needs a positive increment";
-        return NR_E_BATCH;
-    }
-    const int num_digits = dec_digits(2 + (int)((cmax - cmin) / cinc) + (int)((smax - smin) / sinc));
-    FILE *lf = std::fopen(jf.log.c_str(), "w");
-    FILE *ef = std::fopen(jf.edata.c_str(), "w");
-    std::vector<double> u(2 * (size_t)info.n_nodes);
-    const int max_iter = 20 * 2 * info.n_nodes + 1000;
-    int disp_count = 1;
-    for (double comp = cmin; rc == NR_OK && comp < cmax + kFltTol; comp += cinc) {
-        for (double shear = smin; shear < smax + kFltTol; shear += sinc) {
-            nr_linear_report rep;
-            const double sf = affine ? shear / yext : shear;
-            const double cf = affine ? comp / yext : comp;
-            rc = nr_linear_response(net, spec.ks, spec.kb, spec.mu, sf, cf, affine ? 1 : 0, 1e-12,
-                                    max_iter, u.data(), &rep);
-            if (rc) {
-                err = nr_last_error();
-                break;
+    int n_nodes = 0;
+    bool itemize = false;
+    std::vector<double> comps, shears;      // load steps, in the reference's order
+    std::vector<std::vector<double>> u;     // per step: bc + w
+    std::vector<nr_linear_job> jobs;
+    std::vector<nr_linear_report> reps;
+    int digits = 1;
+    ~LinearRun() { nr_network_free(net); }
+
+    int plan(const JobFiles &f, const LinearBatchSpec &spec, bool strain_mode, bool affine,
+             bool item, std::string &err)
+    {
+        jf = &f;
+        itemize = item;
+        start = std::time(nullptr);
+        int rc = nr_network_read(f.dat.c_str(), &net);
+        if (rc) {
+            err = nr_last_error();
+            return rc;
+        }
+        if (spec.kb > 0) nr_network_build_triples(net);
+        nr_network_info info;
+        nr_network_get_info(net, &info);
+        n_nodes = info.n_nodes;
+        std::vector<double> pos(2 * (size_t)info.n_nodes);
+        nr_network_get_arrays(net, pos.data(), nullptr, nullptr, nullptr, nullptr, nullptr, nullptr,
+                              nullptr);
+        double minx = DBL_MAX, maxx = -DBL_MAX;
+        for (int i = 0; i < info.n_nodes; i++) {
+            minx = std::min(minx, pos[2 * i]);
+            maxx = std::max(maxx, pos[2 * i]);
+        }
+        const double yext = info.maxy - info.miny, xext = maxx - minx;
+        // equal ends: one step (run_relaxation_batch :1081-1082), before scaling
+        double cmin = spec.cmin, cmax = spec.cmax, cinc = spec.cinc;
+        double smin = spec.smin, smax = spec.smax, sinc = spec.sinc;
+        if (smax - smin < kFltTol) sinc = 1e12;
+        if (cmax - cmin < kFltTol) cinc = 1e12;
+        if (strain_mode) {
+            cmin *= yext;
+            cmax *= yext;
+            cinc *= yext;
+            smin *= xext;
+            smax *= xext;
+            sinc *= xext;
+        }
+        if (!(cinc > 0) || !(sinc > 0)) {
+            err = "a load range needs a positive increment";
+            return NR_E_BATCH;
+        }
+        digits = dec_digits(2 + (int)((cmax - cmin) / cinc) + (int)((smax - smin) / sinc));
+        const int max_iter = 20 * 2 * info.n_nodes + 1000;
+        for (double comp = cmin; comp < cmax + kFltTol; comp += cinc)
+            for (double shear = smin; shear < smax + kFltTol; shear += sinc) {
+                comps.push_back(comp);
+                shears.push_back(shear);
             }
-            const std::string name = report_name(jf.disp, num_digits, disp_count++, "disp");
+        u.assign(comps.size(), std::vector<double>(2 * (size_t)info.n_nodes));
+        jobs.resize(comps.size());
+        reps.resize(comps.size());
+        for (size_t k = 0; k < comps.size(); k++) {
+            nr_linear_job &jb = jobs[k];
+            jb.net = net;
+            jb.ks = spec.ks;
+            jb.kb = spec.kb;
+            jb.mu = spec.mu;
+            jb.shear = affine ? shears[k] / yext : shears[k];
+            jb.compression = affine ? comps[k] / yext : comps[k];
+            jb.affine = affine ? 1 : 0;
+            jb.tol = 1e-12;
+            jb.max_iter = max_iter;
+            jb.u = u[k].data();
+        }
+        return NR_OK;
+    }
+
+    size_t rows() const { return jobs.size() * 2 * (size_t)n_nodes; }
+
+    void write() const
+    {
+        FILE *lf = std::fopen(jf->log.c_str(), "w");
+        FILE *ef = std::fopen(jf->edata.c_str(), "w");
+        for (size_t k = 0; k < jobs.size(); k++) {
+            const std::string name = report_name(jf->disp, digits, (int)k + 1, "disp");
             FILE *df = std::fopen(name.c_str(), "w");
             if (df) {
-                for (int k = 0; k < info.n_nodes; k++)
-                    std::fprintf(df, "%2.10le\t%2.10le\n", u[2 * k], u[2 * k + 1]);
+                for (int i = 0; i < n_nodes; i++)
+                    std::fprintf(df, "%2.10le\t%2.10le\n", u[k][2 * i], u[k][2 * i + 1]);
                 std::fclose(df);
             }
-            std::fprintf(lf ? lf : stdout, "Residual RMS force: %2.10le\n", rep.residual);
+            std::fprintf(lf ? lf : stdout, "Residual RMS force: %2.10le\n", reps[k].residual);
             FILE *out = ef ? ef : stdout;
             if (!itemize)
-                std::fprintf(out, "%12.8lf\t%12.8lf\t%12.10le\n", comp, shear,
-                             rep.e_stretch + rep.e_bend);
+                std::fprintf(out, "%12.8lf\t%12.8lf\t%12.10le\n", comps[k], shears[k],
+                             reps[k].e_stretch + reps[k].e_bend);
             else
-                std::fprintf(out, "%12.8lf\t%12.8lf\t%12.10le\t%12.10le\n", comp, shear,
-                             rep.e_stretch, rep.e_bend);
+                std::fprintf(out, "%12.8lf\t%12.8lf\t%12.10le\t%12.10le\n", comps[k], shears[k],
+                             reps[k].e_stretch, reps[k].e_bend);
+        }
+        if (ef) std::fclose(ef);
+        if (lf) {
+            std::fprintf(lf, "Elapsed time: %d seconds.\n",
+                         (int)std::difftime(std::time(nullptr), start));
+            std::fclose(lf);
         }
     }
-    if (ef) std::fclose(ef);
-    if (lf) {
-        std::fprintf(lf, "Elapsed time: %d seconds.\n", (int)std::difftime(std::time(nullptr), start));
-        std::fclose(lf);
-    }
-    return rc;
-}
+};
 
 }  // namespace nr
 
-// Networks of a linear batch solved at the same time.
-constexpr size_t kLinearConcurrent = 4;
+// Rows (DOFs x load steps) of one device batch of the linear protocol: a
+// directory's networks go to the device in groups of about this many.
+constexpr size_t kLinearBatchRows = (size_t)1 << 25;
 
 extern "C" int nr_linear_batch(const char *batch_file, int strain_mode, int affine, int itemize)
 {
@@ -695,38 +724,51 @@ extern "C" int nr_linear_batch(const char *batch_file, int strain_mode, int affi
             std::fprintf(stderr, "%s\nFile parsing failed.\n", err.c_str());
             return rc;
         }
-        std::vector<nr::JobFiles> jobs;
-        rc = nr::file_lists(b.base, b.tag, jobs, err);
+        std::vector<nr::JobFiles> files;
+        rc = nr::file_lists(b.base, b.tag, files, err);
         if (rc) {
             std::fprintf(stderr, "%s\nFile lists could not be read.\n", err.c_str());
             return rc;
         }
-        // Networks are independent: a few at a time, each on its own host
-        // thread and device stream, so that one network's dependent CG
-        // launches overlap another's on the device; errors are reported in
-        // the file order afterwards.
-        std::vector<int> rcs(jobs.size(), NR_OK);
-        std::vector<std::string> errs(jobs.size());
-        std::atomic<size_t> next{0};
-        auto worker = [&]() {
-            for (size_t k; (k = next.fetch_add(1)) < jobs.size();)
-                rcs[k] = nr::guarded(NR_E_BATCH, [&]() -> int {
-                    return nr::run_linear_protocol(jobs[k], b, strain_mode != 0, affine != 0,
-                                                   itemize != 0, errs[k]);
-                });
-        };
-        const size_t nw = std::min<size_t>(jobs.size(), kLinearConcurrent);
-        std::vector<std::thread> pool;
-        for (size_t w = 1; w < nw; w++) pool.emplace_back(worker);
-        worker();
-        for (std::thread &t : pool) t.join();
+        // Every load step of every network is an independent solve: the
+        // networks' steps go to the device together, in groups of about
+        // kLinearBatchRows rows, and each network's files are written once
+        // its group is solved; errors are reported in the file order.
         int first_err = NR_OK;
-        for (size_t k = 0; k < jobs.size(); k++)
-            if (rcs[k]) {
-                std::fprintf(stderr, "%s: %s\n", jobs[k].dat.c_str(),
-                             errs[k].empty() ? "failed" : errs[k].c_str());
-                if (!first_err) first_err = rcs[k];
+        auto note = [&](const nr::JobFiles &f, int code, const std::string &msg) {
+            std::fprintf(stderr, "%s: %s\n", f.dat.c_str(), msg.empty() ? "failed" : msg.c_str());
+            if (!first_err) first_err = code;
+        };
+        for (size_t k0 = 0; k0 < files.size();) {
+            std::vector<std::unique_ptr<nr::LinearRun>> group;
+            size_t rows = 0;
+            for (; k0 < files.size() && (group.empty() || rows < kLinearBatchRows); k0++) {
+                auto run = std::make_unique<nr::LinearRun>();
+                std::string e;
+                const int prc = run->plan(files[k0], b, strain_mode != 0, affine != 0,
+                                          itemize != 0, e);
+                if (prc) {
+                    note(files[k0], prc, e);
+                    continue;
+                }
+                rows += run->rows();
+                group.push_back(std::move(run));
             }
+            std::vector<nr_linear_job> jobs;
+            for (auto &run : group) jobs.insert(jobs.end(), run->jobs.begin(), run->jobs.end());
+            std::vector<nr_linear_report> reps(jobs.size());
+            const int src = nr_linear_response_batch((int)jobs.size(), jobs.data(), reps.data());
+            const std::string serr = src ? nr_last_error() : "";
+            size_t at = 0;
+            for (auto &run : group) {
+                std::copy(reps.begin() + at, reps.begin() + at + run->jobs.size(), run->reps.begin());
+                at += run->jobs.size();
+                if (src)
+                    note(*run->jf, src, serr);
+                else
+                    run->write();
+            }
+        }
         return first_err;
     });
 }

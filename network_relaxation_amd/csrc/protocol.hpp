@@ -77,8 +77,5 @@ struct LinearBatchSpec {
     double smin = 0, smax = 0, sinc = 0;
 };
 int parse_linear_batch(const std::string &path, LinearBatchSpec &b, std::string &err);
-// do_relaxation_run of sparse_relax.cpp (:676-881) on the device solve.
-int run_linear_protocol(const JobFiles &jf, const LinearBatchSpec &b, bool strain_mode,
-                        bool affine, bool itemize, std::string &err);
 
 }  // namespace nr

@@ -1,4 +1,5 @@
-"""Device small-strain linear solve (nr_linear_response) against the oracle's
+"""Device small-strain linear solve (nr_linear_response, batched:
+nr_linear_response_batch) against the oracle's
 restatement of the reference's sparse_relax.cpp (itself pinned by
 tests/test_linear.py): energies, residual and the minimum-norm displacement
 field; and the full-lattice closed form on the device."""
@@ -58,10 +59,39 @@ def test_device_linear_repeats_bit_for_bit():
     np.testing.assert_array_equal(a["u"], b["u"])
 
 
+def test_batched_solves_equal_single_solves():
+    """One device batch of systems (networks of different sizes and box
+    modes, load steps sharing a network's matrix, a system that starts
+    converged, one stopped by its iteration cap) gives every system the
+    results of its solve alone, bit for bit."""
+    le = N.Network.lattice(24, 0.7, 3, True).triples().fiber_kappa(1e-4, 1e-2, 3)
+    box = N.Network.lattice(20, 0.75, 4, False).triples()
+    big = N.Network.lattice(40, 0.65, 5, True).triples()
+    full = N.Network.lattice(12, 1.0, 1, True).triples()
+    jobs = [dict(net=le, ks=1.0, kb=0.0, shear=0.01),
+            dict(net=box, ks=1.0, kb=1e-2, shear=0.02, compression=0.01, affine=False),
+            dict(net=le, ks=1.0, kb=0.0, shear=0.03, compression=0.005),
+            dict(net=big, ks=1.0, kb=1e-3, shear=0.01, mu=0.02),
+            dict(net=full, ks=1.0, kb=1e-2, shear=0.02),
+            dict(net=big, ks=1.0, kb=1e-3, shear=0.01, max_iter=40)]
+    print(f"linear batch of {len(jobs)}", flush=True)
+    batch = N.linear_response_batch(jobs, displacements=True)
+    assert batch[4]["iters"] == 0 and batch[5]["iters"] == 40
+    assert batch[5]["status"] == N.NOT_CONVERGED
+    for j, b in zip(jobs, batch):
+        a = N.linear_response(j["net"], j["ks"], j["kb"], j["shear"],
+                              compression=j.get("compression", 0.0), mu=j.get("mu", 0.0),
+                              affine=j.get("affine", True), max_iter=j.get("max_iter", 1000000),
+                              displacements=True)
+        assert (a["iters"], a["status"], a["es"], a["eb"], a["residual"]) == \
+            (b["iters"], b["status"], b["es"], b["eb"], b["residual"])
+        np.testing.assert_array_equal(a["u"], b["u"])
+
+
 def test_concurrent_solves_equal_sequential():
     """Solves from several host threads at once (each on its own device
-    stream, as nrelax_linear runs a batch's networks) give the results of the
-    same solves one at a time, bit for bit."""
+    stream) give the results of the same solves one at a time, bit for
+    bit."""
     from concurrent.futures import ThreadPoolExecutor
     nets = [N.Network.lattice(L, 0.7, s, True).triples().fiber_kappa(1e-4, 1e-2, s)
             for L, s in ((24, 3), (32, 4), (20, 5), (28, 6))]

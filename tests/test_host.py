@@ -89,6 +89,14 @@ def test_linear_response_argument_errors():
     bare = N.Network.lattice(8, 0.8, 1, False)
     with pytest.raises(N.NRError):
         N.linear_response(bare, 1.0, 1e-3, 0.01)
+    # a batch is refused as a whole when any of its jobs is invalid
+    good = N.Network.lattice(8, 0.8, 1, False)
+    with pytest.raises(N.NRError):
+        N.linear_response_batch([dict(net=good, ks=1.0, kb=0.0, shear=0.01),
+                                 dict(net=bare, ks=1.0, kb=1e-3, shear=0.01)])
+    with pytest.raises(N.NRError):
+        N.linear_response_batch([dict(net=good, ks=1.0, kb=0.0, shear=0.01, tol=-1.0)])
+    assert N.linear_response_batch([]) == []
 
 
 def test_linear_cli_rejects_bad_batch_files(tmp_path):
