@@ -101,6 +101,9 @@ def main():
     ap.add_argument("--fcut", type=float, default=1e-9)
     ap.add_argument("--max-steps", type=int, default=3000000)
     ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--block", type=int, default=0, help="threads per workgroup (256 or 512)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="ramp: workgroup / tile shape and tiles per member from nr_autotune")
     ap.add_argument("--team", type=int, default=0)
     ap.add_argument("--tpm", type=int, default=0, help="tiles per team member")
     ap.add_argument("--switch", type=float, default=0.0, help="mixed FP32 -> FP64 hand-over")
@@ -165,7 +168,7 @@ def main():
     nets = build_networks(N, specs)
     progress(f"built {len(nets)} networks")
     prm = N.relax_params(fcut=args.fcut, max_steps=args.max_steps, criterion=N.CRIT_MAX, **FIRE)
-    shape = dict(tile_nodes=args.tile, team=args.team, tiles_per_member=args.tpm)
+    shape = dict(tile_nodes=args.tile, team=args.team, tiles_per_member=args.tpm, block=args.block)
 
     cold_ms = []  # kernel time of the first relax (teams planned from estimated step counts)
 
@@ -207,7 +210,7 @@ def main():
         node_steps += reps[k]["steps"] * n_
     achieved = total_bytes / (kernel_ms * 1e-3) / 1e9
     profiled = (args.networks, args.lmin, args.lmax, args.seed, args.fcut, args.tile, args.team,
-                args.tpm, args.switch) == (512, 16, 192, 2024, 1e-9, 0, 0, 0, 0.0)
+                args.tpm, args.switch, args.block) == (512, 16, 192, 2024, 1e-9, 0, 0, 0, 0.0, 0)
     traffic = PROFILED_TRAFFIC_BYTES if profiled else None
     e_mixed = np.array([r["es"] + r["eb"] for r in reps])
     first_ms = cold_ms[0] if cold_ms else None
@@ -420,9 +423,19 @@ def ramp_leg(N, args):
            "unit": "node-updates/s", "higher_is_better": True,
            "nodes": nodes, "bonds": edges, "triples": tris}
     methods = {"fire": N.METHOD_FIRE, "cg": N.METHOD_CG, "cg_pr": N.METHOD_CG_PR}
+    shape = dict(block=args.block, tile_nodes=args.tile, team=args.team,
+                 tiles_per_member=args.tpm)
+    if args.autotune:
+        t0 = time.perf_counter()
+        tuned, predicted = N.autotune([n], 1.0, 1e-4, precision=N.PREC_FP64, probe_steps=2000,
+                                      team=args.team)
+        shape.update(tuned)
+        out["autotune"] = {"shape": tuned, "predicted_ms": predicted,
+                           "seconds": time.perf_counter() - t0, "probe_steps": 2000}
+        progress(f"ramp: autotune chose {tuned} in {out['autotune']['seconds']:.1f} s")
     for name in args.methods.split(","):
         method = methods[name]
-        b = N.Batch([n], 1.0, 1e-4)
+        b = N.Batch([n], 1.0, 1e-4, **shape)
         prm = N.relax_params(fcut=args.fcut, criterion=N.CRIT_MAX, method=method,
                              max_steps=1000000, **FIRE)
         t = time.perf_counter()
@@ -440,7 +453,9 @@ def ramp_leg(N, args):
         out[name] = {"roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                                   "traffic": RAMP512_TRAFFIC_PER_STEP * steps
-                                  if name == "fire" else None,
+                                  if name == "fire" and not (args.block or args.tile or args.team
+                                                             or args.tpm or args.autotune)
+                                  else None,
                                   "note": "traffic: counted bytes per FIRE step of this "
                                           "network x steps"},
                      "node_updates_per_s": nodes * steps / (kms * 1e-3),
@@ -477,7 +492,8 @@ def large_leg(N, args):
     precs = [x for x in (("mixed", N.PREC_MIXED), ("fp64", N.PREC_FP64))
              if x[0] in args.precisions.split(",")]
     for name, prec in precs:
-        srv = N.Server(precision=prec, slice_steps=20000, hbm_bytes=4 << 30)
+        srv = N.Server(precision=prec, slice_steps=20000, hbm_bytes=4 << 30, block=args.block,
+                       tile_nodes=args.tile, team=args.team)
         t0 = time.perf_counter()
         srv.submit(n, 1.0, 1.0, prm, incs)
         out = []
