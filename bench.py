@@ -557,7 +557,7 @@ def served_leg(N, args):
 
     def run(arrivals):
         srv = N.Server(precision=N.PREC_MIXED, slice_steps=1000,
-                       tiles_per_member=args.served_tpm)
+                       tiles_per_member=args.served_tpm, block=args.block, tile_nodes=args.tile)
         t0 = srv.clock()
         i = done = 0
         res = {}
