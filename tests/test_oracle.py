@@ -342,6 +342,46 @@ def test_cg_quadratic_chain_converges_in_n_steps(pr):
     np.testing.assert_allclose(p[1::2], np.arange(m + 1) * 1.2, atol=1e-11)
 
 
+def _off_direction(s, d):
+    """|component of s orthogonal to d| / |s|."""
+    u = d / np.linalg.norm(d)
+    return np.linalg.norm(s - (s @ u) * u) / np.linalg.norm(s)
+
+
+@pytest.mark.parametrize("pr", [False, True])
+def test_cg_second_direction_is_the_textbook_one(pr):
+    """On a sheared, bent network (energy far from quadratic) the second CG
+    step must lie along F1 + beta d0 with d0 = F0 and the textbook beta:
+    Fletcher-Reeves |F1|^2/|F0|^2, Polak-Ribiere+ max(0, F1.(F1 - F0)/|F0|^2).
+    The points come from the oracle's CG stopped after one and after two
+    iterations and the forces from its force routine (pinned by finite
+    differences); the line search leaves F1.F0 small but not zero, so the
+    two betas give directions 6e-4 apart (relative) while the method's own
+    is met to 1e-13, and the other method's direction (or a sign error in
+    the numerator) is rejected."""
+    n = O.Net.lattice(10, 0.8, 11, True).triples()
+    x0 = n.pos()
+    n.shear(x0, 0.08)
+    ks, kb = 1.0, 5e-2
+    f0 = n.forces(ks, kb, x0)
+    x1 = x0.copy()
+    assert n.cg(ks, kb, x1, fcut=0.0, max_iters=1, pr=pr)["steps"] == 1
+    x2 = x0.copy()
+    assert n.cg(ks, kb, x2, fcut=0.0, max_iters=2, pr=pr)["steps"] == 2
+    f1 = n.forces(ks, kb, x1)
+    # the first step is steepest descent
+    assert _off_direction(x1 - x0, f0) < 1e-10
+    b_fr = (f1 @ f1) / (f0 @ f0)
+    b_pr = max(0.0, f1 @ (f1 - f0) / (f0 @ f0))
+    b_sign = max(0.0, f1 @ (f1 + f0) / (f0 @ f0))
+    assert b_pr > 0.0 and abs(b_fr - b_pr) > 1e-3 * b_fr  # the methods differ here
+    step = x2 - x1
+    want, other = (b_pr, b_fr) if pr else (b_fr, b_pr)
+    assert _off_direction(step, f1 + want * f0) < 1e-9
+    assert _off_direction(step, f1 + other * f0) > 1e-6
+    assert _off_direction(step, f1 + b_sign * f0) > 1e-6
+
+
 # ---- batch file and output names (:188-213, :297-383) ----------------------
 
 def test_report_name_padding():
