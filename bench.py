@@ -122,6 +122,9 @@ def main():
                     help="skip the autotuned, graph-replayed strain step of the batch")
     ap.add_argument("--no-composite", action="store_true",
                     help="skip the network + gel composite relax")
+    ap.add_argument("--no-ramp", action="store_true",
+                    help="skip the two-increment L=512 FIRE ramp (autotuned shape) of the "
+                         "default run")
     ap.add_argument("--composite-L", type=int, default=512)
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="MIB",
@@ -285,6 +288,10 @@ def main():
 
     lin = None if args.no_linear else gpu_linear(N, specs, linear_sample(specs, args))
     comp = None if args.no_composite else composite_leg(N, args)
+    # the single large network (L = 512), its first two shear increments with
+    # the shape nr_autotune picks for it
+    ramp = None if args.no_ramp else ramp_leg(
+        N, argparse.Namespace(**dict(vars(args), increments=2, methods="fire", autotune=True)))
 
     cpu = None
     if not args.no_cpu:
@@ -336,6 +343,7 @@ def main():
         "cpu_baseline": cpu,
         "linear_response_gpu": lin,
         "composite": comp,
+        "ramp_l512": ramp,
     }
     if args.calibrate_copy:  # after the timed work, for the counter passes
         out["copy_calibration"] = calibrate_copy(args.calibrate_copy)
