@@ -52,6 +52,11 @@ PROFILED_TRAFFIC_BYTES = (64 * 182394455 + 128 * 193891664407 +
 # 131,823 steps; the tile blobs stay resident in LDS).
 RAMP512_TRAFFIC_PER_STEP = (64 * 565665115 + 128 * 26990934960 +
                             64 * 25917849604 + 32 * (26457814164 - 25917849604)) / 131823
+# ... and with the shape nr_autotune picks for it (512-thread workgroups on
+# 512-node tiles, 2 tiles per member; profiles/relax_kernel_ramp512_fire_tuned.md)
+RAMP512_TUNED_SHAPE = {"block": 512, "tile_nodes": 512, "tiles_per_member": 2}
+RAMP512_TUNED_TRAFFIC_PER_STEP = (64 * 312728048 + 128 * 22721546895 +
+                                  64 * 25918046213 + 32 * (26188030549 - 25918046213)) / 131824
 FIRE = dict(alpha0=0.1, f_alpha=0.99, dt_max=0.5, f_inc=1.1, f_dec=0.5, nmin=5, mass=1.0)
 
 
@@ -441,6 +446,13 @@ def ramp_leg(N, args):
         out["autotune"] = {"shape": tuned, "predicted_ms": predicted,
                            "seconds": time.perf_counter() - t0, "probe_steps": 2000}
         progress(f"ramp: autotune chose {tuned} in {out['autotune']['seconds']:.1f} s")
+    # counted bytes per FIRE step, for the two shapes whose counters were read
+    explicit = args.block or args.tile or args.team or args.tpm
+    ramp_traffic = None
+    if not explicit and not args.autotune:
+        ramp_traffic = RAMP512_TRAFFIC_PER_STEP
+    elif not explicit and out["autotune"]["shape"] == RAMP512_TUNED_SHAPE:
+        ramp_traffic = RAMP512_TUNED_TRAFFIC_PER_STEP
     for name in args.methods.split(","):
         method = methods[name]
         b = N.Batch([n], 1.0, 1e-4, **shape)
@@ -460,10 +472,8 @@ def ramp_leg(N, args):
         achieved = byt / (kms * 1e-3) / 1e9
         out[name] = {"roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                                   "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                                  "traffic": RAMP512_TRAFFIC_PER_STEP * steps
-                                  if name == "fire" and not (args.block or args.tile or args.team
-                                                             or args.tpm or args.autotune)
-                                  else None,
+                                  "traffic": ramp_traffic * steps
+                                  if name == "fire" and ramp_traffic else None,
                                   "note": "traffic: counted bytes per FIRE step of this "
                                           "network x steps"},
                      "node_updates_per_s": nodes * steps / (kms * 1e-3),
