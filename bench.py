@@ -124,7 +124,8 @@ def main():
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--no-linear", action="store_true", help="skip the GPU linear solves")
     ap.add_argument("--no-graph", action="store_true",
-                    help="skip the autotuned, graph-replayed strain step of the batch")
+                    help="time eager relaxes of the batch (default shape) instead of graph "
+                         "replays of the autotuned batch's strain step")
     ap.add_argument("--no-composite", action="store_true",
                     help="skip the network + gel composite relax")
     ap.add_argument("--no-ramp", action="store_true",
@@ -199,10 +200,55 @@ def main():
             progress(f"timed step {k + 1}/{steps}: {batch.last_ms():.0f} ms")
         return reps, time.perf_counter() - t0, kernel_ms / steps
 
-    # headline: batched + mixed precision
-    mixed = N.Batch(nets, 1.0, 1.0, precision=N.PREC_MIXED, mixed_switch=args.switch, **shape)
+    xd = np.array([s["gamma"] * n.info().height for s, n in zip(specs, nets)])
+
+    def run_graph(batch, steps, warmup):
+        # eager warm-up relaxes (the first planned from estimated step counts),
+        # then the strain step captured and replayed; the first replay must
+        # equal the last eager relax bit for bit
+        cold_ms.clear()
+        eager = None
+        for w in range(max(1, warmup)):
+            batch.reset()
+            batch.strain_all(xd)
+            eager = batch.relax(prm)
+            cold_ms.append(batch.last_ms())
+            progress(f"warmup step {w + 1}/{max(1, warmup)}: {batch.last_ms():.0f} ms")
+        batch.graph_capture(xd, 0.0, prm)
+        kernel_ms = 0.0
+        same = None
+        t0 = time.perf_counter()
+        for k in range(steps):
+            batch.reset()
+            reps = batch.graph_replay()
+            kernel_ms += batch.last_ms()
+            if same is None:
+                same = all((a["steps"], a["es"], a["eb"]) == (b["steps"], b["es"], b["eb"])
+                           for a, b in zip(eager, reps))
+            progress(f"timed step {k + 1}/{steps} (graph replay): {batch.last_ms():.0f} ms")
+        return reps, time.perf_counter() - t0, kernel_ms / steps, same
+
+    # headline: batched + mixed precision, as config 4 states it: shapes from
+    # nr_autotune (unless given) and the timed strain steps replayed from a
+    # captured HIP graph (each one from the unstrained state, after the
+    # untimed eager warm-up relaxes); --no-graph times eager relaxes instead
+    use_graph = not args.no_graph
+    tuned_shape, tune_info = shape, None
+    if use_graph and not (args.tile or args.team or args.tpm or args.block):
+        t0 = time.perf_counter()
+        tuned, predicted = N.autotune(nets, 1.0, 1.0, precision=N.PREC_MIXED, probe_steps=400)
+        tune_info = {"shape": tuned, "predicted_ms": predicted,
+                     "seconds": time.perf_counter() - t0}
+        tuned_shape = dict(shape, **tuned)
+        progress(f"autotune chose {tuned} in {tune_info['seconds']:.1f} s")
+    mixed = N.Batch(nets, 1.0, 1.0, precision=N.PREC_MIXED, mixed_switch=args.switch,
+                    **tuned_shape)
     progress("batched mixed: created")
-    reps, wall, kernel_ms = run_batch(mixed, args.steps, args.warmup)
+    if use_graph:
+        reps, wall, kernel_ms, replay_same = run_graph(mixed, args.steps, args.warmup)
+    else:
+        reps, wall, kernel_ms = run_batch(mixed, args.steps, args.warmup)
+        replay_same = None
     n_conv = int(sum(r["status"] == N.CONVERGED and r["fmax"] < args.fcut for r in reps))
     value = n_conv * args.steps / wall
     t0d = min(r["t_start"] for r in reps)
@@ -224,7 +270,11 @@ def main():
     first_ms = cold_ms[0] if cold_ms else None
     del mixed
 
-    graph = None if args.no_graph else graph_leg(N, nets, specs, prm, args, shape)
+    graph = None if not use_graph else {
+        "timed_steps": "HIP graph replays of the captured strain step (nr_batch_graph_capture, "
+                       "nr_batch_graph_replay), each from the unstrained state",
+        "autotune": tune_info, "shape": tuned_shape,
+        "first_replay_equals_last_eager_relax": replay_same}
 
     # the same batch in FP64 (timed like the mixed form: an untimed first
     # relax, then a relax planned from its step counts), and the mixed form's
@@ -322,7 +372,8 @@ def main():
                                f"|F|max<{args.fcut:g}, mixed precision (FP32 stage on FP64 "
                                f"anchors, FP64 polish)",
                    "networks": len(specs), "nodes": int(sum(s["L"] ** 2 for s in specs)),
-                   "precision": "mixed", "fire": FIRE, "parallelism": "1 GPU, persistent batch"},
+                   "precision": "mixed", "fire": FIRE, "parallelism": "1 GPU, persistent batch",
+                   "timed_steps": "graph replays, autotuned shape" if use_graph else "eager"},
         "converged": n_conv,
         "first_relax": None if first_ms is None else {
             "kernel_ms": first_ms, "networks_per_s": n_conv / (first_ms * 1e-3),
@@ -353,43 +404,6 @@ def main():
     if args.calibrate_copy:  # after the timed work, for the counter passes
         out["copy_calibration"] = calibrate_copy(args.calibrate_copy)
     print(json.dumps(out), flush=True)
-
-
-def graph_leg(N, nets, specs, prm, args, shape):
-    """The batch as its workload states it: shapes from the autotuner
-    (nr_autotune: a probe relax per candidate shape and team rule, step
-    budgets proportional to each network's expected steps, 400 on average)
-    and the strain step captured as a HIP graph (nr_batch_graph_capture) and replayed
-    (nr_batch_graph_replay).  Two eager relaxes first (the second planned from
-    the first's step counts, as the timed headline relaxes are), then the
-    capture of the same step and one timed replay from the unstrained state;
-    the replay must reproduce the eager relax bit for bit."""
-    t0 = time.perf_counter()
-    tuned, predicted = N.autotune(nets, 1.0, 1.0, precision=N.PREC_MIXED, probe_steps=400)
-    t_tune = time.perf_counter() - t0
-    progress(f"graph leg: autotune chose {tuned} (predicted {predicted:.0f} ms) in {t_tune:.1f} s")
-    gb = N.Batch(nets, 1.0, 1.0, precision=N.PREC_MIXED, mixed_switch=args.switch, **tuned)
-    xd = np.array([s["gamma"] * n.info().height for s, n in zip(specs, nets)])
-    eager = None
-    for _ in range(2):
-        gb.reset()
-        gb.strain_all(xd)
-        eager = gb.relax(prm)
-    eager_ms = gb.last_ms()
-    gb.graph_capture(xd, 0.0, prm)
-    gb.reset()
-    reps = gb.graph_replay()
-    ms = gb.last_ms()
-    same = all((a["steps"], a["es"], a["eb"]) == (b["steps"], b["es"], b["eb"])
-               for a, b in zip(eager, reps))
-    n_conv = int(sum(r["status"] == N.CONVERGED and r["fmax"] < args.fcut for r in reps))
-    progress(f"graph leg: replay {ms:.0f} ms (eager {eager_ms:.0f} ms), same as eager: {same}")
-    return {"networks_per_s": n_conv / (ms * 1e-3), "replay_ms": ms, "eager_ms": eager_ms,
-            "converged": n_conv, "replay_equals_eager": bool(same),
-            "autotune": {"shape": tuned, "predicted_ms": predicted, "seconds": t_tune},
-            "note": "one strain step of every network captured as a HIP graph and replayed "
-                    "from the unstrained state, shapes from nr_autotune; device time of the "
-                    "replay (the graph's own copies of the reports included)"}
 
 
 def single_leg(N, args):
