@@ -203,17 +203,20 @@ def main():
     xd = np.array([s["gamma"] * n.info().height for s, n in zip(specs, nets)])
 
     def run_graph(batch, steps, warmup):
-        # eager warm-up relaxes (the first planned from estimated step counts),
-        # then the strain step captured and replayed; the first replay must
-        # equal the last eager relax bit for bit
+        # eager warm-up relaxes (the first planned from estimated step counts,
+        # the others from the last relax's), then the strain step captured and
+        # replayed; the capture keeps the team plan of the last relax, so at
+        # least two relaxes precede it; the first replay must equal the last
+        # eager relax bit for bit
         cold_ms.clear()
         eager = None
-        for w in range(max(1, warmup)):
+        warmup = max(2, warmup)
+        for w in range(warmup):
             batch.reset()
             batch.strain_all(xd)
             eager = batch.relax(prm)
             cold_ms.append(batch.last_ms())
-            progress(f"warmup step {w + 1}/{max(1, warmup)}: {batch.last_ms():.0f} ms")
+            progress(f"warmup step {w + 1}/{warmup}: {batch.last_ms():.0f} ms")
         batch.graph_capture(xd, 0.0, prm)
         kernel_ms = 0.0
         same = None
@@ -274,6 +277,7 @@ def main():
         "timed_steps": "HIP graph replays of the captured strain step (nr_batch_graph_capture, "
                        "nr_batch_graph_replay), each from the unstrained state",
         "autotune": tune_info, "shape": tuned_shape,
+        "eager_warmup_relaxes": max(2, args.warmup),
         "first_replay_equals_last_eager_relax": replay_same}
 
     # the same batch in FP64 (timed like the mixed form: an untimed first
