@@ -181,9 +181,9 @@ int nr_batch_ramp(nr_batch *b, int n_inc, const double *incs,
  * whole step as one graph launch (the host's network records are uploaded
  * first, so steps added with nr_batch_strain and positions set in between
  * are honoured).  Capturing again replaces the graph.  The graph keeps the
- * team plan (class allocation and queue order) of the batch's last relax;
+ * team plan (class allocation and queue order) of the batch's last relax:
  * capture after a relax that was itself planned from known step counts (the
- * second relax of the batch onwards) for the plan of a warm relax. 
+ * second relax of the batch onwards) to replay with the plan of a warm relax.
  * A network relaxed by a team of several workgroups needs the whole team
  * resident at once: eager relaxes launch such teams cooperatively, but a
  * graph node is an ordinary launch whose grid only fits the device's resident
